@@ -1,0 +1,950 @@
+// capi.cpp -- host runtime behind gmapf_c.h: the gmapf::GMAPF object of the
+// reference (include/gmapf/GMAPF.h:30-98, src/GMAPF.cu) re-designed around
+// device-resident SoA state, device-side counts (no per-frame host readback
+// of request or moving-set sizes) and, for the batched form, a captured
+// hipGraph per frame.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gmapf_c.h"
+#include "device.h"
+#include "kernels.h"
+#include "navmesh.h"
+
+using namespace gmapf_amd;
+
+namespace {
+
+struct ApiError : std::runtime_error {
+    int code;
+    ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            throw ApiError(GMAPF_E_DEVICE, std::string("hip error: ") + hipGetErrorString(_e) + \
+                                               " at " #expr);                                \
+    } while (0)
+
+using Clock = std::chrono::steady_clock;
+
+}  // namespace
+
+struct gmapf_s {
+    std::string err;
+    bool inited = false, started = false, step_going = false;
+    gmapf_config cfg{};
+
+    // map build (GMAPF.cu:133-196)
+    std::vector<float> cdt_v;
+    std::vector<int> cdt_e;
+    bool map_filled = false;
+    Navmesh nav;
+
+    // host agent tables; pos is the stable host mirror behind GetAgentPos
+    int n_agents = 0;
+    std::vector<float> pos;
+    std::vector<int> state, curr, targ, gen;
+    std::vector<float> targ_pos;
+    // target changes made after start, applied at the next AsyncStep
+    std::vector<int> upd_ids, upd_targ;
+    std::vector<float> upd_xy;
+
+    // device
+    int dev = 0;
+    hipStream_t s1 = nullptr, s2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_p[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    DevMap dmap{};
+    DevAgents dag{};
+    DevSlots dsl{};
+    DevFrame dfr{};
+    SearchParams sp{};
+    CrowdParams cp{};
+    int search_grid = 0;
+    int persistent = 0;
+    std::vector<void*> dallocs;
+    // pinned staging
+    float* h_step = nullptr;
+    float* h_pos_stage = nullptr;
+    int* h_state_stage = nullptr;
+    unsigned long long* h_stats = nullptr;
+    int* h_upd = nullptr;       // ids | targ
+    float* h_upd_xy = nullptr;
+    int upd_cap = 0;
+    int* d_upd = nullptr;
+    float2* d_upd_xy = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    // profiling
+    Clock::time_point t_step;
+    double last_ms[5] = {0, 0, 0, 0, 0};
+    double sum_ms[5] = {0, 0, 0, 0, 0}, max_ms[5] = {0, 0, 0, 0, 0};
+    long long frames = 0;
+    long long stats_host[8] = {0};
+
+    ~gmapf_s() { release(); }
+
+    template <class T>
+    T* dalloc(size_t n, const char* what) {
+        void* p = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(&p, n * sizeof(T));
+        if (e != hipSuccess)
+            throw ApiError(GMAPF_E_NOMEM, std::string("device allocation failed for ") + what + " (" +
+                                              std::to_string(n * sizeof(T)) + " bytes)");
+        dallocs.push_back(p);
+        return static_cast<T*>(p);
+    }
+
+    void release() {
+        if (!started) return;
+        hipSetDevice(dev);
+        if (s1) hipStreamSynchronize(s1);
+        if (s2) hipStreamSynchronize(s2);
+        if (gexec) hipGraphExecDestroy(gexec);
+        if (graph) hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        for (void* p : dallocs) hipFree(p);
+        dallocs.clear();
+        if (h_step) hipHostFree(h_step);
+        if (h_pos_stage) hipHostFree(h_pos_stage);
+        if (h_state_stage) hipHostFree(h_state_stage);
+        if (h_stats) hipHostFree(h_stats);
+        if (h_upd) hipHostFree(h_upd);
+        if (h_upd_xy) hipHostFree(h_upd_xy);
+        h_step = h_pos_stage = nullptr;
+        h_state_stage = nullptr;
+        h_stats = nullptr;
+        h_upd = nullptr;
+        h_upd_xy = nullptr;
+        for (auto& e : ev_p)
+            if (e) hipEventDestroy(e);
+        if (ev_fork) hipEventDestroy(ev_fork);
+        if (ev_join) hipEventDestroy(ev_join);
+        if (s1) hipStreamDestroy(s1);
+        if (s2) hipStreamDestroy(s2);
+        s1 = s2 = nullptr;
+        started = false;
+    }
+
+    void require_init() const {
+        if (!inited) throw ApiError(GMAPF_E_STATE, "not initialized");
+    }
+
+    // ------------------------------------------------------------ lifecycle
+    void init(const gmapf_config& c) {
+        if (inited) throw ApiError(GMAPF_E_STATE, "already initialized");
+        if (c.agents_max_count < 0) throw ApiError(GMAPF_E_ARG, "AgentsMaxCount must be >= 0");
+        cfg = c;
+        pos.clear();
+        pos.reserve(2 * (size_t)std::max(1, c.agents_max_count));
+        inited = true;
+    }
+
+    void check_config() {
+        int T = cfg.path_finder_threads_per_agents;
+        if (T != 2 && T != 4 && T != 8 && T != 16 && T != 32 && T != 64)
+            throw ApiError(GMAPF_E_ARG, "PathFinderThreadsPerAgents must be one of 2,4,8,16,32,64");
+        int Q = cfg.path_finder_queue_capacity;
+        if (Q < 1 || Q > 256) throw ApiError(GMAPF_E_ARG, "PathFinderQueueCapacity must be in [1, 256]");
+        if (cfg.path_finder_parallel_agents < 1)
+            throw ApiError(GMAPF_E_ARG, "PathFinderParallelAgents must be >= 1");
+        if (!std::isfinite(cfg.path_finder_heuristic_k))
+            throw ApiError(GMAPF_E_ARG, "PathFinderHeuristicK must be finite");
+        if (cfg.step_mode != GMAPF_MODE_PLAIN && cfg.step_mode != GMAPF_MODE_BATCHED)
+            throw ApiError(GMAPF_E_ARG, "unknown step_mode");
+        if (cfg.step_mode == GMAPF_MODE_BATCHED && cfg.round_budget < 1)
+            throw ApiError(GMAPF_E_ARG, "batched mode needs round_budget >= 1");
+    }
+
+    void manual_start() {
+        require_init();
+        if (started) throw ApiError(GMAPF_E_STATE, "already started");
+        if (!map_filled) throw ApiError(GMAPF_E_STATE, "map is not filled");
+        check_config();
+        dev = cfg.device;
+        HIPCHK(hipSetDevice(dev));
+        started = true;  // from here release() frees what was allocated
+        try {
+            start_device();
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+
+    void start_device() {
+        const int V = nav.V;
+        const int N = n_agents;
+        const int P = cfg.path_finder_parallel_agents;
+        const int T = cfg.path_finder_threads_per_agents;
+        const int Q = cfg.path_finder_queue_capacity;
+        HIPCHK(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        for (auto& e : ev_p) HIPCHK(hipEventCreate(&e));
+
+        // ---- navmesh
+        std::vector<int4> nbr4(V);
+        std::vector<float2> cen2(V), tri2(3 * (size_t)V);
+        for (int i = 0; i < V; i++) {
+            nbr4[i] = make_int4(nav.nbr[3 * (size_t)i], nav.nbr[3 * (size_t)i + 1], nav.nbr[3 * (size_t)i + 2], -1);
+            cen2[i] = make_float2(nav.cen[2 * (size_t)i], nav.cen[2 * (size_t)i + 1]);
+            for (int k = 0; k < 3; k++)
+                tri2[3 * (size_t)i + k] = make_float2(nav.tri[6 * (size_t)i + 2 * k], nav.tri[6 * (size_t)i + 2 * k + 1]);
+        }
+        int4* d_nbr = dalloc<int4>(V, "navmesh neighbours");
+        float2* d_cen = dalloc<float2>(V, "navmesh centroids");
+        float2* d_tri = dalloc<float2>(3 * (size_t)V, "navmesh triangles");
+        int* d_cs = dalloc<int>(nav.cell_start.size(), "grid cells");
+        int* d_ct = dalloc<int>(std::max<size_t>(1, nav.cell_tris.size()), "grid lists");
+        int* d_gl = dalloc<int>(std::max<size_t>(1, nav.global_tris.size()), "grid global list");
+        HIPCHK(hipMemcpy(d_nbr, nbr4.data(), V * sizeof(int4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_cen, cen2.data(), V * sizeof(float2), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_tri, tri2.data(), 3 * (size_t)V * sizeof(float2), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_cs, nav.cell_start.data(), nav.cell_start.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!nav.cell_tris.empty())
+            HIPCHK(hipMemcpy(d_ct, nav.cell_tris.data(), nav.cell_tris.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!nav.global_tris.empty())
+            HIPCHK(hipMemcpy(d_gl, nav.global_tris.data(), nav.global_tris.size() * sizeof(int), hipMemcpyHostToDevice));
+        dmap = DevMap{};
+        dmap.V = V;
+        dmap.nbr = d_nbr;
+        dmap.cen = d_cen;
+        dmap.tri = d_tri;
+        dmap.gx0 = nav.gx0;
+        dmap.gy0 = nav.gy0;
+        dmap.inv_cs = nav.inv_cs;
+        dmap.rx0 = nav.rx0;
+        dmap.ry0 = nav.ry0;
+        dmap.rx1 = nav.rx1;
+        dmap.ry1 = nav.ry1;
+        dmap.nx = nav.nx;
+        dmap.ny = nav.ny;
+        dmap.cell_start = d_cs;
+        dmap.cell_tris = d_ct;
+        dmap.global_tris = d_gl;
+        dmap.n_global = (int)nav.global_tris.size();
+
+        // ---- crowd constants (AgentsMover.h:221-224)
+        cp = CrowdParams{};
+        cp.cd = cfg.agent_radius * 2.5f;
+        cp.cd2 = cp.cd * cp.cd;
+        cp.slack = cfg.agent_radius;
+        if (cfg.rot_cs == 0.0f && cfg.rot_sn == 0.0f) {
+            // V2Float.h:39-47 with angle 10: 10 * pi_f / 180.0f in FP32, then
+            // cos/sin evaluated in double and rounded once to FP32.
+            const float pi_f = 3.14159265358979f;
+            float ang = 10.0f * pi_f / 180.0f;
+            cp.rot_cs = (float)std::cos((double)ang);
+            cp.rot_sn = (float)std::sin((double)ang);
+        } else {
+            cp.rot_cs = cfg.rot_cs;
+            cp.rot_sn = cfg.rot_sn;
+        }
+        float cell = cfg.hash_cell > 0 ? cfg.hash_cell : cfg.agent_radius * 2.0f;
+        if (!(cell > 0)) cell = 1.0f;
+        dmap.hx0 = nav.bx0;
+        dmap.hy0 = nav.by0;
+        dmap.hinv = 1.0f / cell;
+        dmap.hnx = (int)((nav.bx1 - nav.bx0) * dmap.hinv) + 1;
+        dmap.hny = (int)((nav.by1 - nav.by0) * dmap.hinv) + 1;
+        if ((long long)dmap.hnx * dmap.hny > (1LL << 26))
+            throw ApiError(GMAPF_E_ARG, "hash_cell too small for the map size");
+        dmap.hk = (int)std::floor(cp.cd / cell) + 1;
+
+        // ---- agents
+        const int Nn = std::max(N, 1);
+        dag = DevAgents{};
+        dag.n = N;
+        dag.pos = dalloc<float2>(Nn, "agent positions");
+        dag.pos_tmp = dalloc<float2>(Nn, "agent positions (tmp)");
+        dag.curr = dalloc<int>(Nn, "agent nodes");
+        dag.targ_pos = dalloc<float2>(Nn, "agent targets");
+        dag.targ = dalloc<int>(Nn, "agent target nodes");
+        dag.state = dalloc<int>(Nn, "agent states");
+        dag.step_idx = dalloc<int>(Nn, "agent step idx");
+        dag.step_node = dalloc<int>(Nn, "agent step node");
+        dag.step_pos = dalloc<float2>(Nn, "agent step pos");
+        dag.path_len = dalloc<int>(Nn, "agent path len");
+        dag.path_cap = V;  // a corridor visits each node at most once
+        dag.path = dalloc<int>((size_t)Nn * V, "agent corridors");
+        dag.gen = dalloc<int>(Nn, "agent gen");
+        dag.inflight = dalloc<int>(Nn, "agent inflight");
+        dag.in_m = dalloc<int>(Nn, "agent moving flag");
+        if (N > 0) {
+            std::vector<float2> tp(N);
+            for (int i = 0; i < N; i++) tp[i] = make_float2(targ_pos[2 * i], targ_pos[2 * i + 1]);
+            HIPCHK(hipMemcpy(dag.pos, pos.data(), N * sizeof(float2), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(dag.targ_pos, tp.data(), N * sizeof(float2), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(dag.curr, curr.data(), N * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(dag.targ, targ.data(), N * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(dag.state, state.data(), N * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(dag.gen, gen.data(), N * sizeof(int), hipMemcpyHostToDevice));
+        }
+        HIPCHK(launch_fill(dag.step_idx, -1, Nn, s1));
+        HIPCHK(launch_fill(dag.step_node, -1, Nn, s1));
+        HIPCHK(launch_fill(dag.path_len, 0, Nn, s1));
+        HIPCHK(launch_fill(dag.inflight, -1, Nn, s1));
+        HIPCHK(launch_fill(dag.in_m, 0, Nn, s1));
+        HIPCHK(hipMemsetAsync(dag.step_pos, 0, Nn * sizeof(float2), s1));
+
+        // ---- search slots
+        dsl = DevSlots{};
+        dsl.P = P;
+        dsl.agent = dalloc<int>(P, "slot agent");
+        dsl.gen = dalloc<int>(P, "slot gen");
+        dsl.S = dalloc<int>(P, "slot start");
+        dsl.Tg = dalloc<int>(P, "slot target");
+        dsl.stamp = dalloc<unsigned>(P, "slot stamp");
+        dsl.status = dalloc<int>(P, "slot status");
+        dsl.mS = dalloc<int>(P, "slot mS");
+        dsl.mT = dalloc<int>(P, "slot mT");
+        dsl.rounds = dalloc<long long>(P, "slot rounds");
+        dsl.expansions = dalloc<long long>(P, "slot expansions");
+        dsl.discoveries = dalloc<long long>(P, "slot discoveries");
+        dsl.vis = dalloc<uint4>((size_t)P * V, "visited tables");
+        dsl.qsave = dalloc<unsigned long long>((size_t)P * T * Q, "parked queues");
+        dsl.qcnt = dalloc<int>((size_t)P * T, "parked queue counts");
+        HIPCHK(launch_fill(dsl.agent, -1, P, s1));
+        HIPCHK(launch_fill(dsl.status, 0, P, s1));
+        HIPCHK(hipMemsetAsync(dsl.stamp, 0, P * sizeof(unsigned), s1));
+        HIPCHK(hipMemsetAsync(dsl.vis, 0, (size_t)P * V * sizeof(uint4), s1));
+        HIPCHK(hipMemsetAsync(dsl.rounds, 0, P * sizeof(long long), s1));
+        HIPCHK(hipMemsetAsync(dsl.expansions, 0, P * sizeof(long long), s1));
+        HIPCHK(hipMemsetAsync(dsl.discoveries, 0, P * sizeof(long long), s1));
+
+        // ---- frame scratch
+        dfr = DevFrame{};
+        const int scan_block = 1024;
+        dfr.flags = dalloc<int>(Nn, "flags");
+        dfr.ranks = dalloc<int>(Nn, "ranks");
+        dfr.sflags = dalloc<int>(P, "slot flags");
+        dfr.sranks = dalloc<int>(P, "slot ranks");
+        dfr.blk = dalloc<int>(Nn / scan_block + 2, "scan partials");
+        dfr.sblk = dalloc<int>(P / scan_block + 2, "slot scan partials");
+        dfr.free_list = dalloc<int>(P, "free slots");
+        dfr.active = dalloc<int>(P, "active slots");
+        dfr.ended = dalloc<int>(P, "ended slots");
+        dfr.counters = dalloc<int>(8, "counters");
+        dfr.stats = dalloc<unsigned long long>(8, "stats");
+        dfr.ncell = dmap.hnx * dmap.hny;
+        dfr.hkey = dalloc<int>(Nn, "hash keys");
+        dfr.hcount = dalloc<int>((size_t)dfr.ncell + 1, "hash counts");
+        dfr.hstart = dalloc<int>((size_t)dfr.ncell + 1, "hash starts");
+        dfr.hfill = dalloc<int>(dfr.ncell, "hash fill");
+        dfr.hscatter = dalloc<int>(Nn, "hash scatter");
+        dfr.horder = dalloc<int>(Nn, "hash order");
+        dfr.hblk = dalloc<int>(dfr.ncell / scan_block + 2, "hash scan partials");
+        dfr.stepv = dalloc<float>(1, "step");
+        HIPCHK(launch_fill(dfr.counters, 0, 8, s1));
+        HIPCHK(launch_fill_u64(dfr.stats, 0, 8, s1));
+        HIPCHK(launch_fill(dfr.hkey, -1, Nn, s1));
+
+        // ---- pinned staging
+        HIPCHK(hipHostMalloc((void**)&h_step, sizeof(float)));
+        HIPCHK(hipHostMalloc((void**)&h_pos_stage, Nn * sizeof(float2)));
+        HIPCHK(hipHostMalloc((void**)&h_state_stage, Nn * sizeof(int)));
+        HIPCHK(hipHostMalloc((void**)&h_stats, 8 * sizeof(unsigned long long)));
+        upd_cap = std::max(N, 1);
+        HIPCHK(hipHostMalloc((void**)&h_upd, 2 * (size_t)upd_cap * sizeof(int)));
+        HIPCHK(hipHostMalloc((void**)&h_upd_xy, 2 * (size_t)upd_cap * sizeof(float)));
+        d_upd = dalloc<int>(2 * (size_t)upd_cap, "target updates");
+        d_upd_xy = dalloc<float2>(upd_cap, "target update positions");
+
+        // ---- search launch shape
+        sp = SearchParams{T, Q, cfg.path_finder_heuristic_k, 0};
+        HIPCHK(search_prepare(T, Q));
+        const int rpw = 64 / T;
+        if (cfg.step_mode == GMAPF_MODE_BATCHED) {
+            int per_cu = 0, cus = 0;
+            HIPCHK(search_occupancy(T, Q, &per_cu));
+            HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            if (per_cu < 1) throw ApiError(GMAPF_E_ARG, "queue capacity does not fit in LDS");
+            int waves_needed = (P + rpw - 1) / rpw;
+            search_grid = std::max(1, std::min(per_cu * cus, waves_needed));
+            persistent = 1;
+            sp.budget = cfg.round_budget;
+        } else {
+            search_grid = (P + rpw - 1) / rpw;
+            persistent = 0;
+            sp.budget = 0;
+        }
+        HIPCHK(hipStreamSynchronize(s1));
+    }
+
+    // ------------------------------------------------------------ frame
+    void enqueue_frame() {
+        HIPCHK(launch_admit(dmap, dag, dsl, dfr, s1));
+        HIPCHK(hipEventRecord(ev_fork, s1));
+        HIPCHK(hipStreamWaitEvent(s2, ev_fork, 0));
+        HIPCHK(launch_search(dmap, dsl, dfr, sp, search_grid, persistent, s1));
+        HIPCHK(launch_finish(dmap, dag, dsl, dfr, s1));
+        HIPCHK(launch_crowd(dmap, dag, dfr, cp, s2));
+        HIPCHK(hipEventRecord(ev_join, s2));
+        HIPCHK(hipStreamWaitEvent(s1, ev_join, 0));
+    }
+
+    void debug_sync(const char* phase) {
+        hipError_t e = hipStreamSynchronize(s1);
+        if (e == hipSuccess) e = hipStreamSynchronize(s2);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess)
+            throw ApiError(GMAPF_E_DEVICE, std::string("hip error after ") + phase + ": " + hipGetErrorString(e));
+    }
+
+    void enqueue_frame_profiled() {
+        HIPCHK(hipEventRecord(ev_p[0], s1));
+        HIPCHK(launch_admit(dmap, dag, dsl, dfr, s1));
+        if (cfg.is_debug) debug_sync("admit");
+        HIPCHK(hipEventRecord(ev_p[1], s1));
+        HIPCHK(launch_search(dmap, dsl, dfr, sp, search_grid, persistent, s1));
+        if (cfg.is_debug) debug_sync("search");
+        HIPCHK(hipEventRecord(ev_p[2], s1));
+        HIPCHK(launch_finish(dmap, dag, dsl, dfr, s1));
+        if (cfg.is_debug) debug_sync("finish");
+        HIPCHK(hipEventRecord(ev_p[3], s1));
+        HIPCHK(launch_crowd(dmap, dag, dfr, cp, s1));
+        if (cfg.is_debug) debug_sync("crowd");
+        HIPCHK(hipEventRecord(ev_p[4], s1));
+    }
+
+    void async_step(float dt) {
+        require_init();
+        if (step_going) return;  // GMAPF.cu:260-261
+        if (!started) manual_start();
+        HIPCHK(hipSetDevice(dev));
+        t_step = Clock::now();
+        *h_step = cfg.agent_speed * dt;  // AgentsMover.h:299
+        HIPCHK(hipMemcpyAsync(dfr.stepv, h_step, sizeof(float), hipMemcpyHostToDevice, s1));
+        if (!upd_ids.empty()) {
+            int n = (int)upd_ids.size();
+            for (int k = 0; k < n; k++) {
+                h_upd[k] = upd_ids[k];
+                h_upd[n + k] = upd_targ[k];
+                h_upd_xy[2 * k] = upd_xy[2 * k];
+                h_upd_xy[2 * k + 1] = upd_xy[2 * k + 1];
+            }
+            HIPCHK(hipMemcpyAsync(d_upd, h_upd, 2 * (size_t)n * sizeof(int), hipMemcpyHostToDevice, s1));
+            HIPCHK(hipMemcpyAsync(d_upd_xy, h_upd_xy, (size_t)n * sizeof(float2), hipMemcpyHostToDevice, s1));
+            HIPCHK(launch_apply_updates(dag, d_upd, d_upd + n, d_upd_xy, n, s1));
+            upd_ids.clear();
+            upd_targ.clear();
+            upd_xy.clear();
+        }
+        if (cfg.step_mode == GMAPF_MODE_BATCHED && !cfg.is_profiler && !cfg.is_debug) {
+            if (!gexec) {
+                HIPCHK(hipStreamBeginCapture(s1, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueue_frame();
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    hipStreamEndCapture(s1, &g);
+                    if (g) hipGraphDestroy(g);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(s1, &graph));
+                HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            }
+            HIPCHK(hipGraphLaunch(gexec, s1));
+        } else if (cfg.is_profiler || cfg.is_debug) {
+            enqueue_frame_profiled();
+        } else {
+            enqueue_frame();
+        }
+        const int N = std::max(n_agents, 1);
+        HIPCHK(hipMemcpyAsync(h_pos_stage, dag.pos, N * sizeof(float2), hipMemcpyDeviceToHost, s1));
+        HIPCHK(hipMemcpyAsync(h_state_stage, dag.state, N * sizeof(int), hipMemcpyDeviceToHost, s1));
+        HIPCHK(hipMemcpyAsync(h_stats, dfr.stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s1));
+        step_going = true;
+    }
+
+    void wait_step_end() {
+        require_init();
+        if (!step_going) return;
+        HIPCHK(hipSetDevice(dev));
+        step_going = false;
+        HIPCHK(hipStreamSynchronize(s1));
+        HIPCHK(hipGetLastError());
+        if (n_agents > 0) {
+            std::memcpy(pos.data(), h_pos_stage, (size_t)n_agents * sizeof(float2));
+            std::memcpy(state.data(), h_state_stage, (size_t)n_agents * sizeof(int));
+        }
+        for (int k = 0; k < 8; k++) stats_host[k] = (long long)h_stats[k];
+        frames++;
+        stats_host[7] = frames;
+        // target changes made while the step ran stay pending for the next step
+        for (int id : upd_ids) state[id] = GMAPF_STATE_SEARCH;
+        last_ms[0] = std::chrono::duration<double, std::milli>(Clock::now() - t_step).count();
+        if (cfg.is_profiler || cfg.is_debug) {
+            for (int k = 0; k < 4; k++) {
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, ev_p[k], ev_p[k + 1]));
+                last_ms[k + 1] = ms;
+            }
+        }
+        for (int k = 0; k < 5; k++) {
+            sum_ms[k] += last_ms[k];
+            max_ms[k] = std::max(max_ms[k], last_ms[k]);
+        }
+    }
+
+    // ------------------------------------------------------------ map
+    void fill_map_start(float w, float h) {
+        require_init();
+        if (started) throw ApiError(GMAPF_E_STATE, "cannot modify map after start");
+        cdt_v = {0.f, 0.f, 0.f, h, w, h, w, 0.f};  // GMAPF.cu:137-144
+        cdt_e = {0, 1, 1, 2, 2, 3, 3, 0};
+    }
+
+    void fill_map_obstacle(const float* xy, int n) {
+        require_init();
+        if (started) throw ApiError(GMAPF_E_STATE, "cannot modify map after start");
+        if (n < 1 || !xy) throw ApiError(GMAPF_E_ARG, "obstacle needs vertices");
+        int start = (int)cdt_v.size() / 2;  // GMAPF.cu:147-159
+        for (int i = 0; i < n; i++) {
+            cdt_v.push_back(xy[2 * i]);
+            cdt_v.push_back(xy[2 * i + 1]);
+            if (i > 0) {
+                cdt_e.push_back(start + i - 1);
+                cdt_e.push_back(start + i);
+            }
+        }
+        cdt_e.push_back(start + n - 1);
+        cdt_e.push_back(start);
+    }
+
+    void fill_map_end() {
+        require_init();
+        if (started) throw ApiError(GMAPF_E_STATE, "cannot modify map after start");
+        if (map_filled) throw ApiError(GMAPF_E_STATE, "map is already filled");
+        if (cdt_v.empty()) throw ApiError(GMAPF_E_STATE, "FillMapStart was not called");
+        std::vector<float> tri6;
+        std::vector<int> nbr3;
+        try {
+            triangulate_map(cdt_v, cdt_e, 1.0f, tri6, nbr3);  // GMAPF.cu:165-169
+        } catch (const std::exception& e) {
+            throw ApiError(GMAPF_E_ARG, std::string("triangulation failed: ") + e.what());
+        }
+        fill_nodes((int)(tri6.size() / 6), tri6.data(), nbr3.data());
+    }
+
+    void fill_nodes(int n, const float* tri6, const int* nbr3) {
+        require_init();
+        if (started) throw ApiError(GMAPF_E_STATE, "cannot modify map after start");
+        if (map_filled) throw ApiError(GMAPF_E_STATE, "map is already filled");
+        try {
+            nav.assign(n, tri6, nbr3);
+        } catch (const std::exception& e) {
+            throw ApiError(GMAPF_E_ARG, e.what());
+        }
+        map_filled = true;
+    }
+
+    // ------------------------------------------------------------ agents
+    int add_agent(float x, float y) {
+        require_init();
+        if (started) throw ApiError(GMAPF_E_STATE, "cannot add agents after start");
+        if (!map_filled) throw ApiError(GMAPF_E_STATE, "map is not filled");
+        if (n_agents >= cfg.agents_max_count) throw ApiError(GMAPF_E_ARG, "AgentsMaxCount reached");
+        int node = nav.locate(x, y);
+        if (node < 0)
+            throw ApiError(GMAPF_E_POS, "initial pos (" + std::to_string(x) + ", " + std::to_string(y) + ") is invalid");
+        pos.push_back(x);
+        pos.push_back(y);
+        state.push_back(GMAPF_STATE_IDLE);
+        curr.push_back(node);
+        targ.push_back(-1);
+        targ_pos.push_back(0.f);
+        targ_pos.push_back(0.f);
+        gen.push_back(0);
+        return n_agents++;
+    }
+
+    void set_target(int id, float x, float y) {
+        require_init();
+        if (!map_filled) throw ApiError(GMAPF_E_STATE, "map is not filled");
+        if (id < 0 || id >= n_agents) throw ApiError(GMAPF_E_ARG, "agent id out of range");
+        int node = nav.locate(x, y);
+        if (node < 0)
+            throw ApiError(GMAPF_E_POS, "target pos (" + std::to_string(x) + ", " + std::to_string(y) + ") is invalid");
+        state[id] = GMAPF_STATE_SEARCH;
+        if (!started) {
+            targ[id] = node;
+            targ_pos[2 * id] = x;
+            targ_pos[2 * id + 1] = y;
+            gen[id]++;
+        } else {
+            upd_ids.push_back(id);
+            upd_targ.push_back(node);
+            upd_xy.push_back(x);
+            upd_xy.push_back(y);
+            if ((int)upd_ids.size() > upd_cap) compact_updates();
+        }
+    }
+
+    // keep only the last update per agent (they would overwrite each other)
+    void compact_updates() {
+        std::vector<int> last(n_agents, -1);
+        for (int k = 0; k < (int)upd_ids.size(); k++) last[upd_ids[k]] = k;
+        std::vector<int> ids, tg;
+        std::vector<float> xy;
+        for (int k = 0; k < (int)upd_ids.size(); k++) {
+            if (last[upd_ids[k]] != k) continue;
+            ids.push_back(upd_ids[k]);
+            tg.push_back(upd_targ[k]);
+            xy.push_back(upd_xy[2 * k]);
+            xy.push_back(upd_xy[2 * k + 1]);
+        }
+        upd_ids.swap(ids);
+        upd_targ.swap(tg);
+        upd_xy.swap(xy);
+    }
+
+    void profiler_dump() const {
+        require_init();
+        if (!cfg.is_profiler) throw ApiError(GMAPF_E_STATE, "profiler disabled");  // GMAPF.cu:328-329
+        const char* names[5] = {"step (wall)", "admit", "search", "finish", "crowd"};
+        std::printf("[gmapf]----------------------\n");
+        for (int k = 0; k < 5; k++)
+            std::printf("[gmapf] %-12s avg: %.4f ms max: %.4f ms\n", names[k],
+                        frames ? sum_ms[k] / frames : 0.0, max_ms[k]);
+        std::printf("[gmapf] found %lld not found %lld expansions %lld rounds %lld\n", stats_host[0],
+                    stats_host[1], stats_host[2], stats_host[5]);
+        std::printf("[gmapf]----------------------\n");
+        std::fflush(stdout);
+    }
+};
+
+// ============================================================== C entry points
+namespace {
+
+template <class F>
+int guarded(gmapf_handle h, F&& f) {
+    if (!h) return GMAPF_E_ARG;
+    try {
+        f();
+        h->err.clear();
+        return GMAPF_OK;
+    } catch (const ApiError& e) {
+        h->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        h->err = "host allocation failed";
+        return GMAPF_E_NOMEM;
+    } catch (const std::exception& e) {
+        h->err = e.what();
+        return GMAPF_E_INTERNAL;
+    }
+}
+
+void check_id(gmapf_handle h, int id) {
+    if (id < 0 || id >= h->n_agents) throw ApiError(GMAPF_E_ARG, "agent id out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
+int gmapf_config_default(gmapf_config* c) {
+    if (!c) return GMAPF_E_ARG;
+    std::memset(c, 0, sizeof(*c));
+    c->agents_max_count = 2048;
+    c->agent_radius = 2.0f;
+    c->agent_speed = 1.0f;
+    c->path_finder_parallel_agents = 2048;
+    c->path_finder_threads_per_agents = 4;
+    c->path_finder_queue_capacity = 16;
+    c->path_finder_heuristic_k = 1.0f;
+    c->step_mode = GMAPF_MODE_PLAIN;
+    c->round_budget = 256;
+    return GMAPF_OK;
+}
+
+int gmapf_create(gmapf_handle* out) {
+    if (!out) return GMAPF_E_ARG;
+    try {
+        *out = new gmapf_s();
+    } catch (...) {
+        return GMAPF_E_NOMEM;
+    }
+    return GMAPF_OK;
+}
+
+int gmapf_destroy(gmapf_handle h) {
+    if (!h) return GMAPF_E_ARG;
+    try {
+        if (h->step_going) h->wait_step_end();
+    } catch (...) {
+    }
+    delete h;
+    return GMAPF_OK;
+}
+
+const char* gmapf_last_error(gmapf_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+int gmapf_init(gmapf_handle h, const gmapf_config* c) {
+    return guarded(h, [&] {
+        if (!c) throw ApiError(GMAPF_E_ARG, "null config");
+        h->init(*c);
+    });
+}
+
+int gmapf_manual_start(gmapf_handle h) { return guarded(h, [&] { h->manual_start(); }); }
+
+int gmapf_fill_map_start(gmapf_handle h, float w, float hh) {
+    return guarded(h, [&] { h->fill_map_start(w, hh); });
+}
+int gmapf_fill_map_obstacle(gmapf_handle h, const float* xy, int n) {
+    return guarded(h, [&] { h->fill_map_obstacle(xy, n); });
+}
+int gmapf_fill_map_end(gmapf_handle h) { return guarded(h, [&] { h->fill_map_end(); }); }
+
+int gmapf_fill_map_manually(gmapf_handle h, const gmapf_node* nodes, int n) {
+    return guarded(h, [&] {
+        if (!nodes || n <= 0) throw ApiError(GMAPF_E_ARG, "no nodes");
+        std::vector<float> tri6(6 * (size_t)n);
+        std::vector<int> nbr3(3 * (size_t)n);
+        for (int i = 0; i < n; i++) {
+            const gmapf_node& nd = nodes[i];
+            float* t = &tri6[6 * (size_t)i];
+            t[0] = nd.p1[0];
+            t[1] = nd.p1[1];
+            t[2] = nd.p2[0];
+            t[3] = nd.p2[1];
+            t[4] = nd.p3[0];
+            t[5] = nd.p3[1];
+            for (int k = 0; k < 3; k++) nbr3[3 * (size_t)i + k] = nd.neibs_idx[k];
+        }
+        h->fill_nodes(n, tri6.data(), nbr3.data());
+    });
+}
+
+int gmapf_add_agent(gmapf_handle h, float x, float y, int* id) {
+    return guarded(h, [&] {
+        int r = h->add_agent(x, y);
+        if (id) *id = r;
+    });
+}
+
+int gmapf_set_agent_targ_pos(gmapf_handle h, int id, float x, float y) {
+    return guarded(h, [&] { h->set_target(id, x, y); });
+}
+
+int gmapf_set_agent_targ_pos_many(gmapf_handle h, int n, const int* ids, const float* xy, int* status) {
+    return guarded(h, [&] {
+        if (n < 0 || (n > 0 && (!ids || !xy))) throw ApiError(GMAPF_E_ARG, "bad arrays");
+        int first_err = 0;
+        std::string msg;
+        for (int k = 0; k < n; k++) {
+            int rc = GMAPF_OK;
+            try {
+                h->set_target(ids[k], xy[2 * k], xy[2 * k + 1]);
+            } catch (const ApiError& e) {
+                rc = e.code;
+                if (!first_err) {
+                    first_err = rc;
+                    msg = e.what();
+                }
+            }
+            if (status) status[k] = rc;
+        }
+        if (first_err && !status) throw ApiError(first_err, msg);
+    });
+}
+
+int gmapf_async_step(gmapf_handle h, float dt) { return guarded(h, [&] { h->async_step(dt); }); }
+int gmapf_wait_step_end(gmapf_handle h) { return guarded(h, [&] { h->wait_step_end(); }); }
+
+int gmapf_get_agent_pos(gmapf_handle h, int id, float* xy) {
+    return guarded(h, [&] {
+        h->require_init();
+        check_id(h, id);
+        if (!xy) throw ApiError(GMAPF_E_ARG, "null output");
+        xy[0] = h->pos[2 * id];
+        xy[1] = h->pos[2 * id + 1];
+    });
+}
+
+int gmapf_get_agent_state(gmapf_handle h, int id, int* st) {
+    return guarded(h, [&] {
+        h->require_init();
+        check_id(h, id);
+        if (!st) throw ApiError(GMAPF_E_ARG, "null output");
+        *st = h->state[id];
+    });
+}
+
+int gmapf_agent_positions(gmapf_handle h, const float** xy, int* n) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (xy) *xy = h->pos.data();
+        if (n) *n = h->n_agents;
+    });
+}
+
+int gmapf_profiler_dump(gmapf_handle h) { return guarded(h, [&] { h->profiler_dump(); }); }
+
+int gmapf_agent_count(gmapf_handle h, int* n) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (n) *n = h->n_agents;
+    });
+}
+
+int gmapf_get_agents_soa(gmapf_handle h, float* xy, int* st, int* node) {
+    return guarded(h, [&] {
+        h->require_init();
+        int N = h->n_agents;
+        if (xy) std::memcpy(xy, h->pos.data(), (size_t)N * 2 * sizeof(float));
+        if (st) std::memcpy(st, h->state.data(), (size_t)N * sizeof(int));
+        if (node) {
+            if (h->started && N > 0) {
+                if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
+                HIPCHK(hipMemcpy(node, h->dag.curr, (size_t)N * sizeof(int), hipMemcpyDeviceToHost));
+            } else if (N > 0) {
+                std::memcpy(node, h->curr.data(), (size_t)N * sizeof(int));
+            }
+        }
+    });
+}
+
+int gmapf_get_agent_corridor(gmapf_handle h, int id, int* buf, int cap, int* len) {
+    return guarded(h, [&] {
+        h->require_init();
+        check_id(h, id);
+        if (!h->started) {
+            if (len) *len = -1;
+            return;
+        }
+        if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
+        int L = 0, st = 0;
+        HIPCHK(hipMemcpy(&L, h->dag.path_len + id, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&st, h->dag.state + id, sizeof(int), hipMemcpyDeviceToHost));
+        if (L <= 0) {
+            if (len) *len = -1;
+            return;
+        }
+        if (len) *len = L;
+        if (buf && cap > 0)
+            HIPCHK(hipMemcpy(buf, h->dag.path + (size_t)id * h->dag.path_cap,
+                             (size_t)std::min(cap, L) * sizeof(int), hipMemcpyDeviceToHost));
+    });
+}
+
+int gmapf_get_agent_steps(gmapf_handle h, int* step_idx, int* step_node, float* step_xy) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!h->started) throw ApiError(GMAPF_E_STATE, "not started");
+        if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
+        int N = h->n_agents;
+        if (N == 0) return;
+        if (step_idx) HIPCHK(hipMemcpy(step_idx, h->dag.step_idx, N * sizeof(int), hipMemcpyDeviceToHost));
+        if (step_node) HIPCHK(hipMemcpy(step_node, h->dag.step_node, N * sizeof(int), hipMemcpyDeviceToHost));
+        if (step_xy) HIPCHK(hipMemcpy(step_xy, h->dag.step_pos, N * sizeof(float2), hipMemcpyDeviceToHost));
+    });
+}
+
+int gmapf_map_size(gmapf_handle h, int* nodes) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (nodes) *nodes = h->map_filled ? h->nav.V : 0;
+    });
+}
+
+int gmapf_get_map(gmapf_handle h, float* tri6, int* nbr3, float* cen2) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!h->map_filled) throw ApiError(GMAPF_E_STATE, "map is not filled");
+        size_t V = h->nav.V;
+        if (tri6) std::memcpy(tri6, h->nav.tri.data(), 6 * V * sizeof(float));
+        if (nbr3) std::memcpy(nbr3, h->nav.nbr.data(), 3 * V * sizeof(int));
+        if (cen2) std::memcpy(cen2, h->nav.cen.data(), 2 * V * sizeof(float));
+    });
+}
+
+int gmapf_locate(gmapf_handle h, int n, const float* xy, int* out) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!h->map_filled) throw ApiError(GMAPF_E_STATE, "map is not filled");
+        for (int i = 0; i < n; i++) out[i] = h->nav.locate(xy[2 * i], xy[2 * i + 1]);
+    });
+}
+
+int gmapf_locate_device(gmapf_handle h, int n, const float* xy, int* out) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!h->started) throw ApiError(GMAPF_E_STATE, "not started");
+        if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
+        if (n <= 0) return;
+        float2* dp = nullptr;
+        int* dout = nullptr;
+        HIPCHK(hipMalloc(&dp, n * sizeof(float2)));
+        hipError_t e = hipMalloc(&dout, n * sizeof(int));
+        if (e == hipSuccess) e = hipMemcpy(dp, xy, n * sizeof(float2), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = launch_locate(h->dmap, dp, n, dout, h->s1);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->s1);
+        if (e == hipSuccess) e = hipMemcpy(out, dout, n * sizeof(int), hipMemcpyDeviceToHost);
+        hipFree(dp);
+        if (dout) hipFree(dout);
+        HIPCHK(e);
+    });
+}
+
+int gmapf_get_stats(gmapf_handle h, long long* out8) {
+    return guarded(h, [&] {
+        h->require_init();
+        for (int k = 0; k < 8; k++) out8[k] = h->stats_host[k];
+    });
+}
+
+int gmapf_last_frame_ms(gmapf_handle h, double* out5) {
+    return guarded(h, [&] {
+        h->require_init();
+        for (int k = 0; k < 5; k++) out5[k] = h->last_ms[k];
+    });
+}
+
+int gmapf_get_hash(gmapf_handle h, int* keys, int* order, int* n) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!h->started) throw ApiError(GMAPF_E_STATE, "not started");
+        if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
+        int N = h->n_agents;
+        int total = 0;
+        HIPCHK(hipMemcpy(&total, h->dfr.hstart + h->dfr.ncell, sizeof(int), hipMemcpyDeviceToHost));
+        if (keys && N) HIPCHK(hipMemcpy(keys, h->dfr.hkey, N * sizeof(int), hipMemcpyDeviceToHost));
+        if (order && total > 0) HIPCHK(hipMemcpy(order, h->dfr.horder, total * sizeof(int), hipMemcpyDeviceToHost));
+        if (n) *n = total;
+    });
+}
+
+int gmapf_get_launch_info(gmapf_handle h, int* out8) {
+    return guarded(h, [&] {
+        h->require_init();
+        out8[0] = h->sp.T;
+        out8[1] = h->sp.Q;
+        out8[2] = h->cfg.path_finder_parallel_agents;
+        out8[3] = h->sp.budget;
+        out8[4] = h->search_grid;
+        out8[5] = h->cfg.step_mode;
+        out8[6] = h->map_filled ? h->nav.V : 0;
+        out8[7] = h->persistent;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// device.h -- device-resident data layouts shared by the host engine and the
+// HIP kernels of the per-frame step.
+//
+// Navmesh (CSR triangle graph, SoA):
+//   nbr[V]   int4  {n0, n1, n2, unused}: compacted neighbours, -1 tail
+//            (reference Node::NeibsIdx, CuNodesMap.h:14-21, GMAPF.cu:185-190)
+//   cen[V]   float2 centroid ((P1+P2)+P3)/3 (GMAPF.cu:215)
+//   tri[V]   3 x float2 vertices, for point-in-triangle tests
+//   point-location grid: cell_start/cell_tris (ascending triangle ids per
+//   cell, triangles listed in every cell their margin-expanded bbox touches)
+//   plus a short list of near-degenerate triangles that is always scanned.
+// Agents (SoA, one entry per agent id):
+//   pos float2, curr int, targ_pos float2, targ int, state int,
+//   step_idx int, step_node int, step_pos float2, path_len int,
+//   path[id * path_cap ...] corridor, gen int (bumped by SetAgentTargPos),
+//   inflight int (search slot or -1), in_m int (moving set of this frame).
+// Search slots (P of them, PathFinderParallelAgents):
+//   vis[slot * V + node] uint4 {stamp, prev, G, dir}; an entry belongs to the
+//   current search of the slot only when its stamp equals slot_stamp[slot],
+//   so slots are never cleared (replaces PathFinder.h:119-136).
+//   qsave[slot][lane][Q] uint64 keys + qcnt[slot][lane]: queues parked
+//   between frames by the batched form.
+#pragma once
+#include <cstdint>
+
+namespace gmapf_amd {
+
+enum AgentState : int { kIdle = 0, kSearch = 1, kMove = 2 };
+
+struct DevMap {
+    int V;
+    const int4* nbr;
+    const float2* cen;
+    const float2* tri;  // 3 per triangle
+    // point location grid
+    float gx0, gy0, inv_cs;
+    float rx0, ry0, rx1, ry1;
+    int nx, ny;
+    const int* cell_start;
+    const int* cell_tris;
+    const int* global_tris;
+    int n_global;
+    // spatial hash (agents), cell keys over the navmesh bbox
+    float hx0, hy0, hinv;
+    int hnx, hny, hk;  // hk = stencil radius in cells
+};
+
+struct DevAgents {
+    int n;
+    float2* pos;
+    float2* pos_tmp;
+    int* curr;
+    float2* targ_pos;
+    int* targ;
+    int* state;
+    int* step_idx;
+    int* step_node;
+    float2* step_pos;
+    int* path_len;
+    int* path;
+    long long path_cap;
+    int* gen;
+    int* inflight;
+    int* in_m;
+};
+
+// Per-slot search record (SoA over P slots).
+struct DevSlots {
+    int P;
+    int* agent;     // -1 free
+    int* gen;       // agent gen at admission
+    int* S;
+    int* Tg;
+    unsigned* stamp;
+    int* status;    // 0 running, 1 found, 2 exhausted, 3 fresh (needs seeding)
+    int* mS;
+    int* mT;
+    long long* rounds;
+    long long* expansions;
+    long long* discoveries;
+    uint4* vis;             // P * V
+    unsigned long long* qsave;  // P * T * Q
+    int* qcnt;              // P * T
+};
+
+// Per-frame scratch / counters (device).
+struct DevFrame {
+    int* flags;      // N   request flags
+    int* ranks;      // N
+    int* sflags;     // P   slot flags
+    int* sranks;     // P
+    int* blk;        // block partials (agents)
+    int* sblk;       // block partials (slots)
+    int* free_list;  // P
+    int* active;     // P: active slot list for the search kernel
+    int* ended;      // P: slots that ended this frame
+    int* counters;   // [0] n_requests [1] n_active [2] work ticket [3] n_ended
+                     // [4] n_free [5] moving count [6] n_admitted
+    unsigned long long* stats;  // [0] found [1] fail [2] expansions [3] discoveries
+                                // [4] corridor nodes [5] rounds [6] admitted
+    // spatial hash
+    int* hkey;        // N
+    int* hcount;      // ncell + 1
+    int* hstart;      // ncell + 1
+    int* hfill;       // ncell
+    int* hscatter;    // N
+    int* horder;      // N
+    int* hblk;        // scan partials over cells
+    int ncell;
+    float* stepv;     // speed * dt of the frame (written by the host before launch)
+};
+
+struct SearchParams {
+    int T;
+    int Q;
+    float K;
+    int budget;  // max rounds per search per launch; <= 0: unbounded
+};
+
+struct CrowdParams {
+    float cd;      // 2.5 * radius
+    float cd2;
+    float slack;   // radius
+    float rot_cs;
+    float rot_sn;
+};
+
+}  // namespace gmapf_amd

@@ -1,0 +1,516 @@
+// k_frame.hip -- per-frame kernels around the search: request admission,
+// corridor build, move, spatial hash (integer counting sort), avoidance and
+// path progress.  Every kernel is deterministic: orders come from prefix sums
+// over agent / slot indices, never from atomic arrival order.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace gmapf_amd {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanBlock = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int block_reduce_sum(int v, int* sh) {
+    // 256 threads = 4 waves
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int w = threadIdx.x / 64, ln = threadIdx.x % 64;
+    __syncthreads();
+    if (ln == 0) sh[w] = v;
+    __syncthreads();
+    int t = 0;
+    if (threadIdx.x == 0) t = sh[0] + sh[1] + sh[2] + sh[3];
+    return t;  // valid in thread 0
+}
+
+// exclusive scan over the block; returns the exclusive prefix of v, *total = block sum
+__device__ __forceinline__ int block_exscan(int v, int* sh, int* total) {
+    const int ln = threadIdx.x % 64, w = threadIdx.x / 64;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (ln >= o) x += y;
+    }
+    __syncthreads();
+    if (ln == 63) sh[w] = x;
+    __syncthreads();
+    int wpre = 0;
+    for (int k = 0; k < w; ++k) wpre += sh[k];
+    *total = sh[0] + sh[1] + sh[2] + sh[3];
+    return wpre + x - v;
+}
+
+// per-block sums of vals[0, n)
+__global__ __launch_bounds__(kScanThreads) void k_block_sum(const int* vals, int n, int* blk) {
+    __shared__ int sh[4];
+    const long long base = (long long)blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) s += vals[base + k];
+    int t = block_reduce_sum(s, sh);
+    if (threadIdx.x == 0) blk[blockIdx.x] = t;
+}
+
+// exclusive scan of vals into out; *total = sum of all
+__global__ __launch_bounds__(kScanThreads) void k_block_scan(const int* vals, int n, const int* blk,
+                                                             int* out, int* total) {
+    __shared__ int sh[4];
+    __shared__ int sh2[4];
+    int o = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) o += blk[b];
+    int off = block_reduce_sum(o, sh);
+    __shared__ int off_sh;
+    if (threadIdx.x == 0) off_sh = off;
+    __syncthreads();
+    off = off_sh;
+    const long long base = (long long)blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? vals[base + k] : 0;
+        s += v[k];
+    }
+    int btot = 0;
+    int pre = block_exscan(s, sh2, &btot);
+    int run = off + pre;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0 && total) *total = off + btot;
+}
+
+hipError_t exscan(const int* vals, int n, int* blk, int* out, int* total, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    int nb = (n + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_block_sum, dim3(nb), dim3(kScanThreads), 0, st, vals, n, blk);
+    hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(kScanThreads), 0, st, vals, n, (const int*)blk,
+                       out, total);
+    return hipGetLastError();
+}
+
+__global__ void k_fill(int* p, int v, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------- geometry
+// CuNodesMap.h:272-275
+__device__ __forceinline__ float get_sign(float p1x, float p1y, float p2x, float p2y, float p3x,
+                                          float p3y) {
+    return (p1x - p3x) * (p2y - p3y) - (p2x - p3x) * (p1y - p3y);
+}
+
+// CuNodesMap.h:260-270
+__device__ __forceinline__ bool in_tri(const DevMap& m, int t, float px, float py) {
+    const float2 a = m.tri[3 * (size_t)t], b = m.tri[3 * (size_t)t + 1], c = m.tri[3 * (size_t)t + 2];
+    float d1 = get_sign(px, py, a.x, a.y, b.x, b.y);
+    float d2 = get_sign(px, py, b.x, b.y, c.x, c.y);
+    float d3 = get_sign(px, py, c.x, c.y, a.x, a.y);
+    bool neg = (d1 < 0) || (d2 < 0) || (d3 < 0);
+    bool pos = (d1 > 0) || (d2 > 0) || (d3 > 0);
+    return !(neg && pos);
+}
+
+__device__ __forceinline__ int cell_of(float v, float o, float inv, int n) {
+    float f = floorf((v - o) * inv);
+    if (!(f >= 0.0f)) return 0;
+    if (f >= (float)(n - 1)) return n - 1;
+    return (int)f;
+}
+
+// CuNodesMap.h:216-229: first (minimum) triangle index containing p, through
+// the margin grid; points outside the gridded region fall back to the scan.
+__device__ int locate(const DevMap& m, float x, float y) {
+    if (!(x >= m.rx0 && x <= m.rx1 && y >= m.ry0 && y <= m.ry1)) {
+        for (int t = 0; t < m.V; ++t)
+            if (in_tri(m, t, x, y)) return t;
+        return -1;
+    }
+    const int cx = cell_of(x, m.gx0, m.inv_cs, m.nx), cy = cell_of(y, m.gy0, m.inv_cs, m.ny);
+    const size_t c = (size_t)cy * m.nx + cx;
+    int best = -1;
+    for (int k = m.cell_start[c], e = m.cell_start[c + 1]; k < e; ++k) {
+        int t = m.cell_tris[k];
+        if (in_tri(m, t, x, y)) {
+            best = t;
+            break;
+        }
+    }
+    for (int k = 0; k < m.n_global; ++k) {
+        int t = m.global_tris[k];
+        if (best >= 0 && t > best) break;
+        if (in_tri(m, t, x, y)) {
+            best = t;
+            break;
+        }
+    }
+    return best;
+}
+
+// ------------------------------------------------------------- admission
+__global__ void k_cancel(DevAgents ag, DevSlots sl, DevFrame fr) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= sl.P) return;
+    int a = sl.agent[s];
+    if (a >= 0 && ag.gen[a] != sl.gen[s]) {  // target changed while in flight
+        sl.agent[s] = -1;
+        ag.inflight[a] = -1;
+        a = -1;
+    }
+    fr.sflags[s] = (a < 0) ? 1 : 0;
+}
+
+// PathFinder.h:85-96 + AgentsMover.h:39-46
+__global__ void k_agent_flags(DevAgents ag, DevFrame fr) {
+    __shared__ int sh[4];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int f = 0, mv = 0;
+    if (i < ag.n) {
+        int st = ag.state[i];
+        mv = (st == kMove) ? 1 : 0;
+        ag.in_m[i] = mv;
+        if (st == kSearch && ag.inflight[i] < 0) {
+            if (ag.curr[i] == ag.targ[i])
+                ag.state[i] = kIdle;
+            else
+                f = 1;
+        }
+        fr.flags[i] = f;
+    }
+    int t = block_reduce_sum(mv, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(&fr.counters[5], t);
+}
+
+__global__ void k_compact(const int* flags, const int* ranks, int n, int* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flags[i]) out[ranks[i]] = i;
+}
+
+// r-th request (ascending agent index) takes the r-th free slot (ascending slot index)
+__global__ void k_admit(DevAgents ag, DevSlots sl, DevFrame fr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ag.n || !fr.flags[i]) return;
+    const int r = fr.ranks[i];
+    if (r >= fr.counters[4]) return;  // no free slot: stays Search (PathFinder.h:97-102)
+    const int s = fr.free_list[r];
+    sl.agent[s] = i;
+    sl.gen[s] = ag.gen[i];
+    sl.S[s] = ag.curr[i];
+    sl.Tg[s] = ag.targ[i];
+    sl.stamp[s] = sl.stamp[s] + 1u;
+    sl.status[s] = 3;
+    sl.rounds[s] = 0;
+    sl.expansions[s] = 0;
+    sl.discoveries[s] = 0;
+    ag.inflight[i] = s;
+    ag.path_len[i] = 0;  // old corridor released (PathFinder.h:104-108)
+    atomicAdd(&fr.stats[6], 1ull);
+}
+
+__global__ void k_slot_active(DevSlots sl, DevFrame fr) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < sl.P) fr.sflags[s] = (sl.agent[s] >= 0) ? 1 : 0;
+    if (s == 0) fr.counters[2] = 0;
+}
+
+// ------------------------------------------------------------- corridor
+// PathFinder.h:253-307, two threads per slot (start side, target side)
+__global__ __launch_bounds__(128) void k_finish(DevMap map, DevAgents ag, DevSlots sl, DevFrame fr) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = tid >> 1;
+    const int side = tid & 1;
+    bool valid = false;
+    int a = -1, status = 0, lenS = 0, lenT = 0;
+    if (s < sl.P) {
+        a = sl.agent[s];
+        status = a >= 0 ? sl.status[s] : 0;
+        valid = a >= 0 && (status == 1 || status == 2);
+    }
+    if (valid && status == 1) {
+        const uint4* vis = sl.vis + (size_t)s * map.V;
+        const int mS = sl.mS[s], mT = sl.mT[s], S = sl.S[s], Tg = sl.Tg[s];
+        lenS = (int)vis[mS].z;
+        lenT = (int)vis[mT].z + 1;
+        int* path = ag.path + (size_t)a * ag.path_cap;
+        if (side == 0) {
+            int idx = lenS - 1, n = mS;
+            while (n != S) {
+                path[idx--] = n;
+                n = (int)vis[n].y;
+            }
+        } else {
+            int idx = lenS, n = mT;
+            while (n != Tg) {
+                path[idx++] = n;
+                n = (int)vis[n].y;
+            }
+            path[idx] = Tg;
+        }
+    }
+    __syncthreads();
+    if (valid && side == 0) {
+        if (status == 1) {
+            ag.path_len[a] = lenS + lenT;
+            ag.state[a] = kMove;
+            ag.step_idx[a] = -1;
+            atomicAdd(&fr.stats[0], 1ull);
+            atomicAdd(&fr.stats[4], (unsigned long long)(lenS + lenT));
+        } else {
+            ag.state[a] = kIdle;
+            atomicAdd(&fr.stats[1], 1ull);
+        }
+        sl.agent[s] = -1;
+        ag.inflight[a] = -1;
+    }
+}
+
+// ------------------------------------------------------------- crowd
+// AgentsMover.h:57-70
+__global__ void k_move(DevAgents ag, DevFrame fr, CrowdParams cp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ag.n || fr.counters[5] == 0) return;
+    if (!ag.in_m[i] || ag.step_idx[i] == -1) return;
+    const float step = *fr.stepv;
+    float2 p = ag.pos[i];
+    const float2 sp = ag.step_pos[i];
+    float dx = sp.x - p.x, dy = sp.y - p.y;
+    float len = sqrtf(dx * dx + dy * dy);
+    if (len <= step) {
+        p = sp;
+    } else {
+        float ux = dx / len, uy = dy / len;
+        p.x = p.x + ux * step;
+        p.y = p.y + uy * step;
+    }
+    ag.pos[i] = p;
+}
+
+__global__ void k_hash_key(DevMap map, DevAgents ag, DevFrame fr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ag.n || fr.counters[5] == 0) return;
+    const float2 p = ag.pos[i];
+    int key = -1;
+    if (!(isnan(p.x) || isnan(p.y))) {
+        int cx = cell_of(p.x, map.hx0, map.hinv, map.hnx);
+        int cy = cell_of(p.y, map.hy0, map.hinv, map.hny);
+        key = cy * map.hnx + cx;
+        atomicAdd(&fr.hcount[key], 1);
+    }
+    fr.hkey[i] = key;
+}
+
+__global__ void k_hash_scatter(DevAgents ag, DevFrame fr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ag.n || fr.counters[5] == 0) return;
+    const int key = fr.hkey[i];
+    if (key < 0) return;
+    int p = fr.hstart[key] + atomicAdd(&fr.hfill[key], 1);
+    fr.hscatter[p] = i;
+}
+
+// stable order inside a cell: rank = number of cell members with a smaller index
+__global__ void k_hash_fixup(DevAgents ag, DevFrame fr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ag.n || fr.counters[5] == 0) return;
+    const int key = fr.hkey[i];
+    if (key < 0) return;
+    const int b = fr.hstart[key], e = fr.hstart[key + 1];
+    int r = 0;
+    for (int p = b; p < e; ++p) r += (fr.hscatter[p] < i) ? 1 : 0;
+    fr.horder[b + r] = i;
+}
+
+// AgentsMover.h:96-126 over the 5x5 (generally (2k+1)^2) stencil, Jacobi, the
+// neighbour terms summed in ascending agent index.
+__global__ void k_avoid(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ag.n || fr.counters[5] == 0) return;
+    const float2 p = ag.pos[i];
+    float sx = 0.f, sy = 0.f;
+    int cnt = 0;
+    const int key = fr.hkey[i];
+    if (key >= 0) {
+        const int cx = key % map.hnx, cy = key / map.hnx;
+        const int x0 = max(0, cx - map.hk), x1 = min(map.hnx - 1, cx + map.hk);
+        const int y0 = max(0, cy - map.hk), y1 = min(map.hny - 1, cy + map.hk);
+        int last = -1;
+        while (true) {
+            int best = INT_MAX;
+            float2 bp = make_float2(0.f, 0.f);
+            for (int yy = y0; yy <= y1; ++yy) {
+                const int row = yy * map.hnx;
+                const int b = fr.hstart[row + x0], e = fr.hstart[row + x1 + 1];
+                for (int q = b; q < e; ++q) {
+                    const int j = fr.horder[q];
+                    if (j <= last || j >= best || j == i) continue;
+                    const float2 o = ag.pos[j];
+                    float dx = p.x - o.x, dy = p.y - o.y;
+                    if (dx * dx + dy * dy < cp.cd2) {
+                        best = j;
+                        bp = o;
+                    }
+                }
+            }
+            if (best == INT_MAX) break;
+            float dx = p.x - bp.x, dy = p.y - bp.y;
+            float dist = sqrtf(dx * dx + dy * dy);
+            float shift = cp.cd - dist;
+            sx += (dx / dist) * shift;
+            sy += (dy / dist) * shift;
+            ++cnt;
+            last = best;
+        }
+    }
+    float2 out = p;
+    if (cnt > 0) {
+        const float fc = (float)cnt;
+        const float ax = sx / fc, ay = sy / fc;
+        const float rx = ax * cp.rot_cs - ay * cp.rot_sn;
+        const float ry = ax * cp.rot_sn + ay * cp.rot_cs;
+        const float nx = p.x + rx, ny = p.y + ry;
+        if (locate(map, nx, ny) >= 0) out = make_float2(nx, ny);
+    }
+    ag.pos_tmp[i] = out;
+}
+
+// AgentsMover.h:137-176 (after the Jacobi write-back)
+__global__ void k_progress(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ag.n || fr.counters[5] == 0) return;
+    const float2 p = ag.pos_tmp[i];
+    ag.pos[i] = p;
+    if (!ag.in_m[i]) return;
+    const int* path = ag.path + (size_t)i * ag.path_cap;
+    int idx = ag.step_idx[i];
+    if (idx == -1) {
+        const int n = path[0];
+        ag.step_idx[i] = 0;
+        ag.step_node[i] = n;
+        ag.step_pos[i] = map.cen[n];
+        return;
+    }
+    const int node = ag.step_node[i];
+    if (!in_tri(map, node, p.x, p.y)) return;
+    ag.curr[i] = node;
+    const int count = ag.path_len[i];
+    if (idx < count) idx += 1;
+    ag.step_idx[i] = idx;
+    if (idx == count) {
+        const float2 t = ag.targ_pos[i];
+        float dx = t.x - p.x, dy = t.y - p.y;
+        if (sqrtf(dx * dx + dy * dy) > FLT_EPSILON + cp.slack)
+            ag.step_pos[i] = t;
+        else
+            ag.state[i] = kIdle;
+    } else {
+        const int n = path[idx];
+        ag.step_node[i] = n;
+        ag.step_pos[i] = map.cen[n];
+    }
+}
+
+// GMAPF.cu:247-250 for target changes made after start
+__global__ void k_apply_updates(DevAgents ag, const int* ids, const int* targ, const float2* xy, int n) {
+    // one thread handles all updates in order so repeated ids keep the last one
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int k = 0; k < n; ++k) {
+        const int i = ids[k];
+        ag.state[i] = kSearch;
+        ag.targ[i] = targ[k];
+        ag.targ_pos[i] = xy[k];
+        ag.gen[i] = ag.gen[i] + 1;
+    }
+}
+
+__global__ void k_locate(DevMap map, const float2* pts, int n, int* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = locate(map, pts[i].x, pts[i].y);
+}
+
+inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
+
+}  // namespace
+
+hipError_t launch_fill(int* p, int v, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(n, 256)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& sl,
+                        const DevFrame& fr, hipStream_t st) {
+    (void)map;
+    hipError_t e;
+    if ((e = launch_fill(fr.counters, 0, 8, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cancel, dim3(nblk(sl.P, 256)), dim3(256), 0, st, ag, sl, fr);
+    hipLaunchKernelGGL(k_agent_flags, dim3(nblk(ag.n, 256)), dim3(256), 0, st, ag, fr);
+    if ((e = exscan(fr.flags, ag.n, fr.blk, fr.ranks, &fr.counters[0], st)) != hipSuccess) return e;
+    if ((e = exscan(fr.sflags, sl.P, fr.sblk, fr.sranks, &fr.counters[4], st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_compact, dim3(nblk(sl.P, 256)), dim3(256), 0, st, (const int*)fr.sflags,
+                       (const int*)fr.sranks, sl.P, fr.free_list);
+    hipLaunchKernelGGL(k_admit, dim3(nblk(ag.n, 256)), dim3(256), 0, st, ag, sl, fr);
+    hipLaunchKernelGGL(k_slot_active, dim3(nblk(sl.P, 256)), dim3(256), 0, st, sl, fr);
+    if ((e = exscan(fr.sflags, sl.P, fr.sblk, fr.sranks, &fr.counters[1], st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_compact, dim3(nblk(sl.P, 256)), dim3(256), 0, st, (const int*)fr.sflags,
+                       (const int*)fr.sranks, sl.P, fr.active);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish(const DevMap& map, const DevAgents& ag, const DevSlots& sl,
+                         const DevFrame& fr, hipStream_t st) {
+    hipLaunchKernelGGL(k_finish, dim3(nblk(2LL * sl.P, 128)), dim3(128), 0, st, map, ag, sl, fr);
+    return hipGetLastError();
+}
+
+hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& fr,
+                        const CrowdParams& cp, hipStream_t st) {
+    hipError_t e;
+    const int nb = nblk(ag.n, 256);
+    hipLaunchKernelGGL(k_move, dim3(nb), dim3(256), 0, st, ag, fr, cp);
+    if ((e = launch_fill(fr.hcount, 0, fr.ncell + 1, st)) != hipSuccess) return e;
+    if ((e = launch_fill(fr.hfill, 0, fr.ncell, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_hash_key, dim3(nb), dim3(256), 0, st, map, ag, fr);
+    if ((e = exscan(fr.hcount, fr.ncell, fr.hblk, fr.hstart, &fr.hstart[fr.ncell], st)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_hash_scatter, dim3(nb), dim3(256), 0, st, ag, fr);
+    hipLaunchKernelGGL(k_hash_fixup, dim3(nb), dim3(256), 0, st, ag, fr);
+    hipLaunchKernelGGL(k_avoid, dim3(nb), dim3(256), 0, st, map, ag, fr, cp);
+    hipLaunchKernelGGL(k_progress, dim3(nb), dim3(256), 0, st, map, ag, fr, cp);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_updates(const DevAgents& ag, const int* ids, const int* targ,
+                                const float2* xy, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_apply_updates, dim3(1), dim3(64), 0, st, ag, ids, targ, xy, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_locate(const DevMap& map, const float2* pts, int n, int* out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_locate, dim3(nblk(n, 256)), dim3(256), 0, st, map, pts, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace gmapf_amd

@@ -1,0 +1,41 @@
+// kernels.h -- host-callable launchers for the per-frame HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace gmapf_amd {
+
+// k_search.hip
+int search_lds_bytes(int Q);
+hipError_t launch_search(const DevMap& map, const DevSlots& slots, const DevFrame& fr,
+                         const SearchParams& sp, int grid, int persistent, hipStream_t st);
+hipError_t search_occupancy(int T, int Q, int* blocks_per_cu);
+// sets the dynamic LDS limit of the search kernel for (T, Q); call before launching/capturing
+hipError_t search_prepare(int T, int Q);
+
+// k_frame.hip
+// Fill n ints with v (used instead of hipMemsetAsync so frames can be captured).
+hipError_t launch_fill(int* p, int v, long long n, hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, long long n, hipStream_t st);
+
+// F1: cancel stale searches, mark requests / moving set, admit in ascending
+// agent index into free slots in ascending slot index, build the active list.
+hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& slots,
+                        const DevFrame& fr, hipStream_t st);
+// F3 tail: corridors for found searches, Idle for exhausted, slots freed.
+hipError_t launch_finish(const DevMap& map, const DevAgents& ag, const DevSlots& slots,
+                         const DevFrame& fr, hipStream_t st);
+// F4: move, hash + avoidance (Jacobi), progress; all no-ops when the moving
+// set of the frame is empty.
+hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& fr,
+                        const CrowdParams& cp, hipStream_t st);
+
+// target changes made by the host after start (state Search, new target, gen+1)
+hipError_t launch_apply_updates(const DevAgents& ag, const int* ids, const int* targ,
+                                const float2* xy, int n, hipStream_t st);
+
+// bulk point location (min containing triangle or -1)
+hipError_t launch_locate(const DevMap& map, const float2* pts, int n, int* out, hipStream_t st);
+
+}  // namespace gmapf_amd
