@@ -573,17 +573,20 @@ void world_step(World& w, float dt) {
     long long budget = w.cfg.round_budget > 0 ? w.cfg.round_budget : (1LL << 62);
     for (auto*& s : w.searches) {
         bool ended = false;
+        long long e0 = s->expansions, d0 = s->discoveries, r0 = s->rounds;
         for (long long r = 0; r < budget; r++) {
             if (search_round(m, *s)) {
                 ended = true;
                 break;
             }
         }
+        // work is counted in the frame it is done (also for searches that
+        // are later dropped or still running)
+        w.st[2] += s->expansions - e0;
+        w.st[3] += s->discoveries - d0;
+        w.st[5] += s->rounds - r0;
         if (!ended) continue;
         Agent& a = w.agents[s->agent];
-        w.st[2] += s->expansions;
-        w.st[3] += s->discoveries;
-        w.st[5] += s->rounds;
         if (s->found) {  // PathFinder.h:280-290
             build_corridor(*s, a.path);
             a.has_path = true;
