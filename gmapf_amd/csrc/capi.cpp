@@ -317,13 +317,14 @@ struct gmapf_s {
         dsl.rounds = dalloc<long long>(P, "slot rounds");
         dsl.expansions = dalloc<long long>(P, "slot expansions");
         dsl.discoveries = dalloc<long long>(P, "slot discoveries");
-        dsl.vis = dalloc<uint4>((size_t)P * V, "visited tables");
+        dsl.vis = dalloc<uint2>((size_t)P * V, "visited tables");
+        dsl.par = dalloc<int>((size_t)P * V, "parent tables");
         dsl.qsave = dalloc<unsigned long long>((size_t)P * T * Q, "parked queues");
         dsl.qcnt = dalloc<int>((size_t)P * T, "parked queue counts");
         HIPCHK(launch_fill(dsl.agent, -1, P, s1));
         HIPCHK(launch_fill(dsl.status, 0, P, s1));
         HIPCHK(hipMemsetAsync(dsl.stamp, 0, P * sizeof(unsigned), s1));
-        HIPCHK(hipMemsetAsync(dsl.vis, 0, (size_t)P * V * sizeof(uint4), s1));
+        HIPCHK(hipMemsetAsync(dsl.vis, 0, (size_t)P * V * sizeof(uint2), s1));
         HIPCHK(hipMemsetAsync(dsl.rounds, 0, P * sizeof(long long), s1));
         HIPCHK(hipMemsetAsync(dsl.expansions, 0, P * sizeof(long long), s1));
         HIPCHK(hipMemsetAsync(dsl.discoveries, 0, P * sizeof(long long), s1));
@@ -392,8 +393,7 @@ struct gmapf_s {
         HIPCHK(launch_admit(dmap, dag, dsl, dfr, s1));
         HIPCHK(hipEventRecord(ev_fork, s1));
         HIPCHK(hipStreamWaitEvent(s2, ev_fork, 0));
-        HIPCHK(launch_search(dmap, dsl, dfr, sp, search_grid, persistent, s1));
-        HIPCHK(launch_finish(dmap, dag, dsl, dfr, s1));
+        HIPCHK(launch_search(dmap, dag, dsl, dfr, sp, search_grid, persistent, s1));
         HIPCHK(launch_crowd(dmap, dag, dfr, cp, s2));
         HIPCHK(hipEventRecord(ev_join, s2));
         HIPCHK(hipStreamWaitEvent(s1, ev_join, 0));
@@ -412,11 +412,9 @@ struct gmapf_s {
         HIPCHK(launch_admit(dmap, dag, dsl, dfr, s1));
         if (cfg.is_debug) debug_sync("admit");
         HIPCHK(hipEventRecord(ev_p[1], s1));
-        HIPCHK(launch_search(dmap, dsl, dfr, sp, search_grid, persistent, s1));
+        HIPCHK(launch_search(dmap, dag, dsl, dfr, sp, search_grid, persistent, s1));
         if (cfg.is_debug) debug_sync("search");
         HIPCHK(hipEventRecord(ev_p[2], s1));
-        HIPCHK(launch_finish(dmap, dag, dsl, dfr, s1));
-        if (cfg.is_debug) debug_sync("finish");
         HIPCHK(hipEventRecord(ev_p[3], s1));
         HIPCHK(launch_crowd(dmap, dag, dfr, cp, s1));
         if (cfg.is_debug) debug_sync("crowd");
@@ -432,6 +430,7 @@ struct gmapf_s {
         *h_step = cfg.agent_speed * dt;  // AgentsMover.h:299
         HIPCHK(hipMemcpyAsync(dfr.stepv, h_step, sizeof(float), hipMemcpyHostToDevice, s1));
         if (!upd_ids.empty()) {
+            compact_updates();  // one update per agent: the last one wins
             int n = (int)upd_ids.size();
             for (int k = 0; k < n; k++) {
                 h_upd[k] = upd_ids[k];

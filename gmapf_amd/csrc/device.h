@@ -15,7 +15,8 @@
 //   path[id * path_cap ...] corridor, gen int (bumped by SetAgentTargPos),
 //   inflight int (search slot or -1), in_m int (moving set of this frame).
 // Search slots (P of them, PathFinderParallelAgents):
-//   vis[slot * V + node] uint4 {stamp, prev, G, dir}; an entry belongs to the
+//   vis[slot * V + node] uint2 {stamp, G << 1 | dir} and par[slot * V + node]
+//   (parent node, read only by the corridor walk); an entry belongs to the
 //   current search of the slot only when its stamp equals slot_stamp[slot],
 //   so slots are never cleared (replaces PathFinder.h:119-136).
 //   qsave[slot][lane][Q] uint64 keys + qcnt[slot][lane]: queues parked
@@ -78,7 +79,8 @@ struct DevSlots {
     long long* rounds;
     long long* expansions;
     long long* discoveries;
-    uint4* vis;             // P * V
+    uint2* vis;             // P * V  {stamp, G << 1 | dir}
+    int* par;               // P * V  parent node
     unsigned long long* qsave;  // P * T * Q
     int* qcnt;              // P * T
 };

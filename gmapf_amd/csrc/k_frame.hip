@@ -228,57 +228,6 @@ __global__ void k_slot_active(DevSlots sl, DevFrame fr) {
     if (s == 0) fr.counters[2] = 0;
 }
 
-// ------------------------------------------------------------- corridor
-// PathFinder.h:253-307, two threads per slot (start side, target side)
-__global__ __launch_bounds__(128) void k_finish(DevMap map, DevAgents ag, DevSlots sl, DevFrame fr) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = tid >> 1;
-    const int side = tid & 1;
-    bool valid = false;
-    int a = -1, status = 0, lenS = 0, lenT = 0;
-    if (s < sl.P) {
-        a = sl.agent[s];
-        status = a >= 0 ? sl.status[s] : 0;
-        valid = a >= 0 && (status == 1 || status == 2);
-    }
-    if (valid && status == 1) {
-        const uint4* vis = sl.vis + (size_t)s * map.V;
-        const int mS = sl.mS[s], mT = sl.mT[s], S = sl.S[s], Tg = sl.Tg[s];
-        lenS = (int)vis[mS].z;
-        lenT = (int)vis[mT].z + 1;
-        int* path = ag.path + (size_t)a * ag.path_cap;
-        if (side == 0) {
-            int idx = lenS - 1, n = mS;
-            while (n != S) {
-                path[idx--] = n;
-                n = (int)vis[n].y;
-            }
-        } else {
-            int idx = lenS, n = mT;
-            while (n != Tg) {
-                path[idx++] = n;
-                n = (int)vis[n].y;
-            }
-            path[idx] = Tg;
-        }
-    }
-    __syncthreads();
-    if (valid && side == 0) {
-        if (status == 1) {
-            ag.path_len[a] = lenS + lenT;
-            ag.state[a] = kMove;
-            ag.step_idx[a] = -1;
-            atomicAdd(&fr.stats[0], 1ull);
-            atomicAdd(&fr.stats[4], (unsigned long long)(lenS + lenT));
-        } else {
-            ag.state[a] = kIdle;
-            atomicAdd(&fr.stats[1], 1ull);
-        }
-        sl.agent[s] = -1;
-        ag.inflight[a] = -1;
-    }
-}
-
 // ------------------------------------------------------------- crowd
 // AgentsMover.h:57-70
 __global__ void k_move(DevAgents ag, DevFrame fr, CrowdParams cp) {
@@ -424,17 +373,16 @@ __global__ void k_progress(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp
     }
 }
 
-// GMAPF.cu:247-250 for target changes made after start
+// GMAPF.cu:247-250 for target changes made after start; the host passes at
+// most one update per agent (the last one), so the updates are independent.
 __global__ void k_apply_updates(DevAgents ag, const int* ids, const int* targ, const float2* xy, int n) {
-    // one thread handles all updates in order so repeated ids keep the last one
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int k = 0; k < n; ++k) {
-        const int i = ids[k];
-        ag.state[i] = kSearch;
-        ag.targ[i] = targ[k];
-        ag.targ_pos[i] = xy[k];
-        ag.gen[i] = ag.gen[i] + 1;
-    }
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = ids[k];
+    ag.state[i] = kSearch;
+    ag.targ[i] = targ[k];
+    ag.targ_pos[i] = xy[k];
+    ag.gen[i] = ag.gen[i] + 1;
 }
 
 __global__ void k_locate(DevMap map, const float2* pts, int n, int* out) {
@@ -477,12 +425,6 @@ hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& 
     return hipGetLastError();
 }
 
-hipError_t launch_finish(const DevMap& map, const DevAgents& ag, const DevSlots& sl,
-                         const DevFrame& fr, hipStream_t st) {
-    hipLaunchKernelGGL(k_finish, dim3(nblk(2LL * sl.P, 128)), dim3(128), 0, st, map, ag, sl, fr);
-    return hipGetLastError();
-}
-
 hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& fr,
                         const CrowdParams& cp, hipStream_t st) {
     hipError_t e;
@@ -503,7 +445,7 @@ hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& 
 hipError_t launch_apply_updates(const DevAgents& ag, const int* ids, const int* targ,
                                 const float2* xy, int n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_apply_updates, dim3(1), dim3(64), 0, st, ag, ids, targ, xy, n);
+    hipLaunchKernelGGL(k_apply_updates, dim3(nblk(n, 256)), dim3(256), 0, st, ag, ids, targ, xy, n);
     return hipGetLastError();
 }
 

@@ -1,11 +1,11 @@
 // k_search.hip -- bidirectional multi-lane bounded-queue search on gfx950.
 //
-// Semantics (reference: include/gmapf/PathFinder.h:138-251, misc/CuQueue.h:33-70,
+// Semantics (reference: include/gmapf/PathFinder.h:138-307, misc/CuQueue.h:33-70,
 // misc/CuVisitedMap.h:45-54, misc/CuNodesMap.h:237-243):
 //   * a search owns T lanes; lane 0 is seeded with the start node (dir 1),
 //     lane 1 with the target node (dir 0), both with F = dist2(S,Tg)*K;
 //   * a round: every lane pops the minimum of its own queue and expands up to
-//     three neighbours (compacted list, stops at the first -1) with
+//     three neighbours (stops at the first -1) with
 //     h = dist2(centroid(n), centroid(opposite endpoint)), G' = G+1,
 //     F' = (float)G' + h*K;
 //   * claims are resolved in (lane, slot) order: the first candidate that
@@ -14,16 +14,26 @@
 //     and the first meet in (lane, slot) order wins;
 //   * frontier item i is pushed to the queue of lane i mod T; queues keep the
 //     Q smallest items by (F, node);
-//   * the search ends when a meet was found or every queue is empty.
+//   * the search ends when a meet was found or every queue is empty; a found
+//     search writes the corridor [start side reversed, start excluded] ++
+//     [target side up to the target] into the agent's corridor buffer.
 // The reference resolves claims and meets with atomics in arbitrary order;
 // here the order is fixed, so results are deterministic and equal the
 // oracle's serial restatement.
 //
-// Mapping: one workgroup = one wave64 = 64/T searches.  Queues live in LDS
-// (entry i of lane L at q[i*64+L], 64-bit keys (F,node) that compare as
-// unsigned integers).  Claim prefix sums and the first-meet choice are wave
-// ballots + popcounts; duplicate claims inside a round are found through a
-// small LDS candidate table.  The visited table is per slot, stamp-tagged.
+// Mapping: one workgroup = one wave64 = 64/T searches ("segments").  Each
+// lane's queue is a binary min-heap of 64-bit keys (F bits made monotone,
+// node id) in LDS, entry i of lane L at q[i*64+L] (bank-conflict free);
+// when full, a smaller key replaces the largest leaf and sifts up.  Claim
+// prefix sums and the first-meet choice are wave ballots + popcounts;
+// duplicate claims inside a round are found by a vectorised scan of a small
+// LDS candidate table.  Visited entries are 8 bytes {stamp, G<<1|dir} in a
+// per-slot table (stamp = search generation, so tables are never cleared);
+// parents go to a separate array read only by the corridor walk.
+//
+// Forms: plain (one search per segment, run to completion) and persistent
+// (segments take searches from an atomic ticket over the active list, run up
+// to `budget` rounds, park their queues in HBM and take the next ticket).
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -50,27 +60,62 @@ __device__ __forceinline__ float dist2(float2 a, float2 b) {
     return x * x + y * y;
 }
 
-// Bounded queue stored descending (q[0] = max, q[cnt-1] = min), CuQueue.h:33-70
-// with (F, node) ordering.
-__device__ __forceinline__ void q_push(unsigned long long* q, int L, int& cnt, int Q,
-                                       unsigned long long k) {
-    if (cnt < Q) {
-        int i = cnt - 1;
-        while (i >= 0) {
-            unsigned long long v = q[i * kWave + L];
-            if (!(v < k)) break;
-            q[(i + 1) * kWave + L] = v;
-            --i;
-        }
-        q[(i + 1) * kWave + L] = k;
-        ++cnt;
-    } else {
-        int p = 0;
-        while (p < cnt && q[p * kWave + L] > k) ++p;
-        if (p == 0) return;  // not smaller than the maximum: dropped
-        for (int i = 0; i < p - 1; ++i) q[i * kWave + L] = q[(i + 1) * kWave + L];
-        q[(p - 1) * kWave + L] = k;
+// ---- per-lane binary min-heap in LDS, "keep the Q smallest" (CuQueue.h:33-70)
+__device__ __forceinline__ void heap_sift_up(unsigned long long* q, int L, int i, unsigned long long k) {
+    while (i > 0) {
+        const int p = (i - 1) >> 1;
+        const unsigned long long v = q[p * kWave + L];
+        if (v < k) break;
+        q[i * kWave + L] = v;
+        i = p;
     }
+    q[i * kWave + L] = k;
+}
+
+__device__ __forceinline__ void heap_push(unsigned long long* q, int L, int& cnt, int Q, unsigned long long k) {
+    if (cnt < Q) {
+        heap_sift_up(q, L, cnt, k);
+        ++cnt;
+        return;
+    }
+    // full: the maximum is a leaf (indices Q/2 .. Q-1)
+    int mi = Q >> 1;
+    unsigned long long mv = q[mi * kWave + L];
+    for (int i = mi + 1; i < Q; ++i) {
+        const unsigned long long v = q[i * kWave + L];
+        if (v > mv) {
+            mv = v;
+            mi = i;
+        }
+    }
+    if (k > mv) return;  // not smaller than the maximum: dropped
+    heap_sift_up(q, L, mi, k);
+}
+
+__device__ __forceinline__ unsigned long long heap_pop(unsigned long long* q, int L, int& cnt) {
+    const unsigned long long top = q[L];
+    --cnt;
+    const unsigned long long last = q[cnt * kWave + L];
+    int i = 0;
+    while (true) {
+        const int l = 2 * i + 1;
+        if (l >= cnt) break;
+        const int r = l + 1;
+        unsigned long long cv = q[l * kWave + L];
+        int c = l;
+        if (r < cnt) {
+            const unsigned long long rv = q[r * kWave + L];
+            if (rv < cv) {
+                cv = rv;
+                c = r;
+            }
+        }
+        if (!(cv < last)) break;
+        q[i * kWave + L] = cv;
+        i = c;
+    }
+    if (cnt > 0) q[i * kWave + L] = last;
+    return top;
 }
 
 template <int T>
@@ -82,34 +127,43 @@ __device__ __forceinline__ unsigned long long seg_mask(int sub) {
     }
 }
 
+// candidate table: 3T entries per segment, padded to a multiple of 4
 template <int T>
-__global__ __launch_bounds__(64) void k_search(DevMap map, DevSlots slots, DevFrame fr,
+constexpr int cand_stride() {
+    return ((3 * T + 3) / 4) * 4;
+}
+
+template <int T>
+__global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlots slots, DevFrame fr,
                                                SearchParams sp, int persistent) {
     extern __shared__ unsigned long long q[];  // 64 * Q keys
-    __shared__ int cand[kWave * 3];
+    constexpr int RPW = kWave / T;
+    constexpr int CS = cand_stride<T>();
+    __shared__ __align__(16) int cand[RPW * CS];
     __shared__ unsigned long long stage[kWave * 3];
 
     const int L = threadIdx.x;
     const int sub = L / T;
     const int l = L % T;
-    constexpr int RPW = kWave / T;
     const int segbase = sub * 3 * T;
+    const int cbase = sub * CS;
     const unsigned long long smask = seg_mask<T>(sub);
     const unsigned long long below = smask & ((1ull << L) - 1ull);
     const int Q = sp.Q;
     const size_t V = (size_t)map.V;
     const int n_active = fr.counters[1];
 
-    // per-segment request state (replicated in every lane of the segment)
+    // per-segment search state (replicated in every lane of the segment)
     int slot = -1;
-    bool has_req = false, no_more = false;
+    bool has_req = false, no_more = false, took_plain = false;
     int S = 0, Tg = 0, cnt = 0;
     unsigned stamp = 0;
     float2 cS = make_float2(0.f, 0.f), cT = make_float2(0.f, 0.f);
     int rounds_here = 0;
     long long rounds_tot = 0, exp_here = 0, disc_here = 0;
     int status = 0;
-    bool took_plain = false;
+    uint2* vis = slots.vis;
+    int* par = slots.par;
 
     while (true) {
         // ---- acquire work for segments that have none
@@ -136,24 +190,25 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevSlots slots, DevFr
                 rounds_here = 0;
                 exp_here = 0;
                 disc_here = 0;
-                uint4* vis = slots.vis + (size_t)slot * V;
+                vis = slots.vis + (size_t)slot * V;
+                par = slots.par + (size_t)slot * V;
                 if (status == 3) {
                     // PathFinder.h:168-189
-                    float F0 = dist2(cS, cT) * sp.K;
+                    const float F0 = dist2(cS, cT) * sp.K;
                     cnt = 0;
                     if (l == 0) {
                         q[L] = make_key(F0, S);
                         cnt = 1;
-                        vis[S] = make_uint4(stamp, 0xffffffffu, 0u, 1u);
+                        vis[S] = make_uint2(stamp, 1u);  // G 0, dir 1
                     }
                     if (l == 1) {
                         q[L] = make_key(F0, Tg);
                         cnt = 1;
-                        vis[Tg] = make_uint4(stamp, 0xffffffffu, 0u, 0u);
+                        vis[Tg] = make_uint2(stamp, 0u);  // G 0, dir 0
                     }
                     status = 0;
                 } else {
-                    size_t qb = ((size_t)slot * T + l);
+                    const size_t qb = ((size_t)slot * T + l);
                     cnt = slots.qcnt[qb];
                     const unsigned long long* src = slots.qsave + qb * (size_t)Q;
                     for (int i = 0; i < cnt; ++i) q[i * kWave + L] = src[i];
@@ -162,25 +217,22 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevSlots slots, DevFr
             }
         }
         if (__ballot(has_req) == 0ull) break;
-        __syncthreads();  // seeds / restored queues and visited writes visible
+        __syncthreads();  // seeds, restored queues and visited writes visible
 
-        // ---- one round
-        uint4* vis = slots.vis + (size_t)slot * V;
+        // ---- pop + neighbour fetch
         const bool popped = has_req && cnt > 0;
         int c = -1, G = 0, cdir = 0;
-        int n0 = -1, n1 = -1, n2 = -1;
+        int nn[3] = {-1, -1, -1};
         if (popped) {
-            --cnt;
-            c = key_node(q[cnt * kWave + L]);
-            int4 nb = map.nbr[c];
-            uint4 vc = vis[c];
-            G = (int)vc.z;
-            cdir = (int)vc.w;
-            n0 = nb.x;
-            n1 = n0 >= 0 ? nb.y : -1;
-            n2 = n1 >= 0 ? nb.z : -1;
+            c = key_node(heap_pop(q, L, cnt));
+            const int4 nb = map.nbr[c];
+            const uint2 vc = vis[c];
+            G = (int)(vc.y >> 1);
+            cdir = (int)(vc.y & 1u);
+            nn[0] = nb.x;
+            nn[1] = nn[0] >= 0 ? nb.y : -1;
+            nn[2] = nn[1] >= 0 ? nb.z : -1;
         }
-        int nn[3] = {n0, n1, n2};
         float Fn[3];
         bool prior[3];
         int pdir[3];
@@ -190,42 +242,54 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevSlots slots, DevFr
             prior[s] = false;
             pdir[s] = 0;
             if (nn[s] >= 0) {
-                float2 cn = map.cen[nn[s]];
-                uint4 vn = vis[nn[s]];
-                float h = cdir ? dist2(cn, cT) : dist2(cn, cS);
+                const float2 cn = map.cen[nn[s]];
+                const uint2 vn = vis[nn[s]];
+                const float h = cdir ? dist2(cn, cT) : dist2(cn, cS);
                 Fn[s] = (float)(G + 1) + h * sp.K;
                 prior[s] = (vn.x == stamp);
-                pdir[s] = (int)vn.w;
+                pdir[s] = (int)(vn.y & 1u);
             }
         }
 #pragma unroll
-        for (int s = 0; s < 3; ++s) cand[segbase + l * 3 + s] = nn[s] >= 0 ? (nn[s] * 2 + cdir) : -1;
+        for (int s = 0; s < 3; ++s) cand[cbase + l * 3 + s] = nn[s] >= 0 ? (nn[s] * 2 + cdir) : -1;
+        if (l == T - 1 && CS > 3 * T) {
+            for (int k = 3 * T; k < CS; ++k) cand[cbase + k] = -1;
+        }
         __syncthreads();
 
+        // ---- duplicate claims of this round: first earlier candidate with the same node
+        int first[3] = {-1, -1, -1};
+        if (popped) {
+            const int4* c4 = reinterpret_cast<const int4*>(&cand[cbase]);
+            const int nchunk = (3 * l + 2 + 3) >> 2;
+            for (int ch = 0; ch < nchunk; ++ch) {
+                const int4 v = c4[ch];
+                const int vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int k = ch * 4 + e;
+#pragma unroll
+                    for (int s = 0; s < 3; ++s) {
+                        if (first[s] < 0 && k < 3 * l + s && vv[e] >= 0 && (vv[e] >> 1) == nn[s]) first[s] = vv[e];
+                    }
+                }
+            }
+        }
         bool claim[3], meet[3];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
             claim[s] = false;
             meet[s] = false;
             if (nn[s] < 0) continue;
-            int first = -1;
-            const int kk = l * 3 + s;
-            for (int k = 0; k < kk; ++k) {
-                int cv = cand[segbase + k];
-                if (cv >= 0 && (cv >> 1) == nn[s]) {
-                    first = cv;
-                    break;
-                }
-            }
-            if (!prior[s] && first < 0) {
+            if (!prior[s] && first[s] < 0) {
                 claim[s] = true;
             } else {
-                int odir = prior[s] ? pdir[s] : (first & 1);
+                const int odir = prior[s] ? pdir[s] : (first[s] & 1);
                 meet[s] = (odir != cdir);
             }
         }
 
-        // claims: frontier order = (lane, slot) order
+        // ---- claims: frontier order = (lane, slot) order
         const int cl = (int)claim[0] + (int)claim[1] + (int)claim[2];
         const unsigned long long b0 = __ballot(cl & 1);
         const unsigned long long b1 = __ballot(cl & 2);
@@ -237,26 +301,30 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevSlots slots, DevFr
             for (int s = 0; s < 3; ++s) {
                 if (claim[s]) {
                     stage[segbase + f] = make_key(Fn[s], nn[s]);
-                    vis[nn[s]] = make_uint4(stamp, (unsigned)c, (unsigned)(G + 1), (unsigned)cdir);
+                    vis[nn[s]] = make_uint2(stamp, ((unsigned)(G + 1) << 1) | (unsigned)cdir);
+                    par[nn[s]] = c;
                     ++f;
                 }
             }
         }
-        // first meet in (lane, slot) order
+        // ---- first meet in (lane, slot) order
         const int ms = meet[0] ? 0 : (meet[1] ? 1 : (meet[2] ? 2 : 3));
         const unsigned long long mb = __ballot(ms < 3) & smask;
         const bool found = has_req && (mb != 0ull);
-        if (found && L == __ffsll((long long)mb) - 1) {
-            int n = nn[ms];
-            slots.mS[slot] = cdir ? c : n;
-            slots.mT[slot] = cdir ? n : c;
+        int mS = -1, mT = -1;
+        {
+            const int w = found ? __ffsll((long long)mb) - 1 : sub * T;
+            const int n = ms < 3 ? nn[ms] : -1;
+            const int a = cdir ? c : n, b = cdir ? n : c;
+            mS = __shfl(a, w, kWave);
+            mT = __shfl(b, w, kWave);
         }
         const unsigned long long pb = __ballot(popped) & smask;
         __syncthreads();
 
-        // deal frontier item f to lane f mod T
+        // ---- deal frontier item f to lane f mod T
         if (has_req) {
-            for (int f = l; f < tot; f += T) q_push(q, L, cnt, Q, stage[segbase + f]);
+            for (int f = l; f < tot; f += T) heap_push(q, L, cnt, Q, stage[segbase + f]);
         }
         const unsigned long long nb = __ballot(cnt > 0) & smask;
         if (has_req) {
@@ -271,33 +339,64 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevSlots slots, DevFr
             }
             const bool ended = status != 0;
             const bool park = !ended && sp.budget > 0 && rounds_here >= sp.budget;
-            if (ended || park) {
-                if (park) {
-                    size_t qb = ((size_t)slot * T + l);
-                    slots.qcnt[qb] = cnt;
-                    unsigned long long* dst = slots.qsave + qb * (size_t)Q;
-                    for (int i = 0; i < cnt; ++i) dst[i] = q[i * kWave + L];
-                }
-                if (l == 0) {
-                    slots.status[slot] = status;
-                    slots.rounds[slot] = rounds_tot;
-                    slots.expansions[slot] += exp_here;
-                    slots.discoveries[slot] += disc_here;
-                    atomicAdd(&fr.stats[2], (unsigned long long)exp_here);
-                    atomicAdd(&fr.stats[3], (unsigned long long)disc_here);
-                    atomicAdd(&fr.stats[5], (unsigned long long)rounds_here);
-                }
-                has_req = false;
+            if (park) {
+                const size_t qb = ((size_t)slot * T + l);
+                slots.qcnt[qb] = cnt;
+                unsigned long long* dst = slots.qsave + qb * (size_t)Q;
+                for (int i = 0; i < cnt; ++i) dst[i] = q[i * kWave + L];
             }
+            if (ended && status == 1 && l < 2) {
+                // PathFinder.h:276-306: lane 0 writes the start side, lane 1 the target side
+                const int a = slots.agent[slot];
+                int* path = ag.path + (size_t)a * ag.path_cap;
+                const int lenS = (int)(vis[mS].y >> 1);
+                if (l == 0) {
+                    int idx = lenS - 1, n = mS;
+                    while (n != S) {
+                        path[idx--] = n;
+                        n = par[n];
+                    }
+                } else {
+                    int idx = lenS, n = mT;
+                    while (n != Tg) {
+                        path[idx++] = n;
+                        n = par[n];
+                    }
+                    path[idx] = Tg;
+                }
+            }
+            if (l == 0 && (ended || park)) {
+                slots.status[slot] = status;
+                slots.rounds[slot] = rounds_tot;
+                slots.expansions[slot] += exp_here;
+                slots.discoveries[slot] += disc_here;
+                atomicAdd(&fr.stats[2], (unsigned long long)exp_here);
+                atomicAdd(&fr.stats[3], (unsigned long long)disc_here);
+                atomicAdd(&fr.stats[5], (unsigned long long)rounds_here);
+                if (ended) {
+                    const int a = slots.agent[slot];
+                    if (status == 1) {  // PathFinder.h:280-290
+                        const int len = (int)(vis[mS].y >> 1) + (int)(vis[mT].y >> 1) + 1;
+                        slots.mS[slot] = mS;
+                        slots.mT[slot] = mT;
+                        ag.path_len[a] = len;
+                        ag.state[a] = kMove;
+                        ag.step_idx[a] = -1;
+                        atomicAdd(&fr.stats[0], 1ull);
+                        atomicAdd(&fr.stats[4], (unsigned long long)len);
+                    } else {  // PathFinder.h:268-274
+                        ag.state[a] = kIdle;
+                        atomicAdd(&fr.stats[1], 1ull);
+                    }
+                    slots.agent[slot] = -1;
+                    ag.inflight[a] = -1;
+                }
+            }
+            if (ended || park) has_req = false;
         }
     }
 }
 
-}  // namespace
-
-int search_lds_bytes(int Q) { return kWave * Q * (int)sizeof(unsigned long long); }
-
-namespace {
 const void* search_fn(int T) {
     switch (T) {
         case 2: return (const void*)k_search<2>;
@@ -309,7 +408,10 @@ const void* search_fn(int T) {
         default: return nullptr;
     }
 }
+
 }  // namespace
+
+int search_lds_bytes(int Q) { return kWave * Q * (int)sizeof(unsigned long long); }
 
 hipError_t search_prepare(int T, int Q) {
     const void* fn = search_fn(T);
@@ -317,13 +419,13 @@ hipError_t search_prepare(int T, int Q) {
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, search_lds_bytes(Q));
 }
 
-hipError_t launch_search(const DevMap& map, const DevSlots& slots, const DevFrame& fr,
+hipError_t launch_search(const DevMap& map, const DevAgents& ag, const DevSlots& slots, const DevFrame& fr,
                          const SearchParams& sp, int grid, int persistent, hipStream_t st) {
-    size_t lds = (size_t)search_lds_bytes(sp.Q);
-#define GMAPF_SEARCH_CASE(TT)                                                                      \
-    case TT:                                                                                       \
-        hipLaunchKernelGGL(k_search<TT>, dim3(grid), dim3(kWave), lds, st, map, slots, fr, sp,     \
-                           persistent);                                                            \
+    const size_t lds = (size_t)search_lds_bytes(sp.Q);
+#define GMAPF_SEARCH_CASE(TT)                                                                        \
+    case TT:                                                                                         \
+        hipLaunchKernelGGL(k_search<TT>, dim3(grid), dim3(kWave), lds, st, map, ag, slots, fr, sp,   \
+                           persistent);                                                              \
         return hipGetLastError();
     switch (sp.T) {
         GMAPF_SEARCH_CASE(2)

@@ -8,7 +8,8 @@ namespace gmapf_amd {
 
 // k_search.hip
 int search_lds_bytes(int Q);
-hipError_t launch_search(const DevMap& map, const DevSlots& slots, const DevFrame& fr,
+// F3: search + corridor build + agent state for ended searches (PathFinder.h:138-307)
+hipError_t launch_search(const DevMap& map, const DevAgents& ag, const DevSlots& slots, const DevFrame& fr,
                          const SearchParams& sp, int grid, int persistent, hipStream_t st);
 hipError_t search_occupancy(int T, int Q, int* blocks_per_cu);
 // sets the dynamic LDS limit of the search kernel for (T, Q); call before launching/capturing
@@ -23,9 +24,6 @@ hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, long lon
 // agent index into free slots in ascending slot index, build the active list.
 hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& slots,
                         const DevFrame& fr, hipStream_t st);
-// F3 tail: corridors for found searches, Idle for exhausted, slots freed.
-hipError_t launch_finish(const DevMap& map, const DevAgents& ag, const DevSlots& slots,
-                         const DevFrame& fr, hipStream_t st);
 // F4: move, hash + avoidance (Jacobi), progress; all no-ops when the moving
 // set of the frame is empty.
 hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& fr,

@@ -70,6 +70,10 @@ def test_multi_frame_world_parity(oracle, mode, budget, P):
         H.compare_worlds(g, w, f, check_corridors=(f % 10 == 0))
         if f == 60:  # re-target a few agents mid-run, including searching ones
             for i in range(0, 160, 7):
+                if i % 2 == 0:  # two changes in one frame: the last one wins
+                    x0, y0 = targs[(i * 5) % 160]
+                    g.set_agent_targ_pos(i, x0, y0)
+                    assert w.set_target(i, x0, y0) == 0
                 x, y = targs[(i * 13) % 160]
                 g.set_agent_targ_pos(i, x, y)
                 assert w.set_target(i, x, y) == 0
@@ -83,13 +87,14 @@ def test_determinism_and_batched_equals_plain_corridors(oracle):
     corridors as the plain form for the same requests."""
     tri, nbr, cfg, starts, targs = _world(33, 32, 32, 96, 4, 16)
     runs = []
-    for mode in (G.MODE_PLAIN, G.MODE_PLAIN, G.MODE_BATCHED):
+    for mode, frames in ((G.MODE_PLAIN, 40), (G.MODE_PLAIN, 40), (G.MODE_BATCHED, 400)):
         cfg.step_mode = mode
         cfg.round_budget = 2
         g = H.make_gpu(tri, nbr, cfg, starts, targs)
-        for _ in range(40):
+        for _ in range(frames):
             g.step(1.0 / 60)
         xy, st = g.agents()
+        assert not (st == G.SEARCH).any()  # every frame-0 request has ended
         cors = [g.corridor(i) for i in range(len(starts))]
         runs.append((xy.copy(), st.copy(), cors))
         g.close()
