@@ -213,7 +213,10 @@ def bowtie_obstacles():
     return [np.array([(30, 40), (30, 60), (60, 40), (60, 60)], np.float32)]
 
 
-def rect_obstacles(seed, W, H, n, smin, smax, gap=1.0):
+def rect_obstacles(seed, W, H, n, smin, smax, gap=2.0):
+    """n axis-aligned rectangles, sides U[smin, smax], at least `gap` apart and
+    from the map border (2 units keeps every vertex clear of the triangulator's
+    1-unit snap tolerance)."""
     r = stream(seed, 21)
     rects = []
     tries = 0
