@@ -153,14 +153,32 @@ def roofline_search(sc, T, Q, budget, frames=6):
     g.close()
     gbs = tot_bytes / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else 0.0
     return dict(bound="hbm", achieved=round(gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(gbs / HBM_PEAK_GBS, 5), traffic=None,
+                frac=round(gbs / HBM_PEAK_GBS, 5), traffic=measured_traffic(T, Q, budget, sc.n),
                 kernel="k_search (persistent)", bytes_per_launch=tot_bytes / max(1, frames - 2),
                 ms_per_launch=tot_ms / max(1, frames - 2))
 
 
+def measured_traffic(T, Q, budget, n):
+    """HBM bytes per k_search launch from the committed rocprofv3 counter
+    summary, when it was taken with the same (T, Q, budget, agents)."""
+    path = os.path.join(ROOT, "profiles", "k_search_maze16k_pmc.md")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    want = f"profile_config: T={T} Q={Q} budget={budget} agents={n}"
+    if want not in txt:
+        return None
+    for line in txt.splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(sc, T, Q, seconds=12.0):
     """Single-core oracle (the CPU restatement) on a bounded sample of the
-    same request distribution: searches until ~`seconds` of CPU time."""
+    same request stream: every agent's start node with its 1st, 2nd, ...
+    log-uniform target, searched until ~`seconds` of CPU time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
@@ -168,19 +186,26 @@ def cpu_baseline(sc, T, Q, seconds=12.0):
     tri, nbr, _ = g.map_arrays()
     om = pyoracle.OracleMap(tri, nbr)
     S_nodes = g.locate(sc.starts)
-    T_nodes = g.locate(sc.targets)
-    found = 0
-    n = 0
+    counters = np.array(sc.counters, copy=True)
+    targets = sc.targets
+    found = n = rep = 0
     t0 = time.process_time()
-    while time.process_time() - t0 < seconds and n < sc.n:
-        if S_nodes[n] != T_nodes[n]:
-            ok, _, _ = om.search(int(S_nodes[n]), int(T_nodes[n]), T, Q, sc.K)
-            found += int(ok)
-        n += 1
+    while time.process_time() - t0 < seconds:
+        T_nodes = g.locate(targets)
+        for i in range(sc.n):
+            if time.process_time() - t0 >= seconds:
+                break
+            if S_nodes[i] != T_nodes[i]:
+                ok, _, _ = om.search(int(S_nodes[i]), int(T_nodes[i]), T, Q, sc.K)
+                found += int(ok)
+                n += 1
+        rep += 1
+        targets = S.log_uniform_targets(sc.seed, 302, sc.starts, 10.0, min(4000.0, sc.width), sc.width, sc.width,
+                                        sc.locate, counters)
     el = time.process_time() - t0
     return dict(value=round(found / el, 2), unit="solves/s", cores=1, kind="port",
-                sample=f"first {n} of the {sc.n} frame-0 requests of the maze workload, single-threaded oracle, "
-                       f"{el:.1f} s CPU")
+                sample=f"{n} searches ({rep} passes over the {sc.n} agents' start nodes with log-uniform targets) "
+                       f"of the maze workload, single-threaded oracle, {el:.1f} s CPU")
 
 
 def main():
