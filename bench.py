@@ -6,7 +6,8 @@ Workload (headline): the 4096x4096 spanning-tree maze (64-unit cells,
 ended (continuous re-path load).  The same request stream runs through the
 plain step (every admitted search runs to completion inside its frame) and
 the batched step (persistent work-queue search with a per-frame round budget,
-frame replayed from a captured hipGraph), both with the same (T, Q).
+frame replayed from a captured hipGraph), both with the same (T, Q): the
+autotuned pair from gmapf_amd/tuned.json when present.
 
 Metric: path solves/s (found searches / summed frame wall time, frame =
 AsyncStep entry -> WaitStepEnd return) of the batched step; p50/p99 frame ms,
@@ -30,106 +31,48 @@ sys.path.insert(0, ROOT)
 
 import gmapf_amd as G  # noqa: E402
 from gmapf_amd import scenarios as S  # noqa: E402
+from gmapf_amd.autotune import TUNE_FILE  # noqa: E402
+from gmapf_amd.runner import Locators, make_world, retarget, run_frames  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-TUNE_FILE = os.path.join(ROOT, "gmapf_amd", "tuned.json")
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_locator(W, H, obs):
-    g = G.GMAPF()
-    g.init(G.Config(agents_max_count=1))
-    g.fill_map_start(W, H)
-    for o in obs:
-        g.fill_map_obstacle(o)
-    g.fill_map_end()
-    make_locator.keep.append(g)
-    return lambda p: g.locate(p)
-
-
-make_locator.keep = []
-
-
-def tuned_pair(name, n):
+def tuned_triple(name, n):
     try:
         with open(TUNE_FILE) as f:
-            d = json.load(f)
-        e = d[f"{name}/{n}"]
+            e = json.load(f)[f"{name}/{n}"]
         return int(e["T"]), int(e["Q"]), int(e["budget"])
     except (OSError, KeyError, ValueError):
         return None
 
 
-def run_mode(sc, mode, T, Q, budget, warmup, steps, profile=False):
-    cfg = G.Config(agents_max_count=sc.n, agent_radius=sc.radius, agent_speed=sc.speed,
-                   path_finder_parallel_agents=sc.P, path_finder_threads_per_agents=T,
-                   path_finder_queue_capacity=Q, path_finder_heuristic_k=sc.K,
-                   step_mode=mode, round_budget=budget, is_profiler=profile)
-    g = G.GMAPF()
-    g.init(cfg)
-    sc.build(g)
-    for x, y in sc.starts:
-        g.add_agent(x, y)
-    ids = np.arange(sc.n, dtype=np.int32)
-    g.set_agent_targets(ids, sc.targets)
-    counters = np.array(sc.counters, copy=True)
-    g.manual_start()
-    W = sc.width
-    frame_ms, found_d, exp_d, search_ms = [], [], [], []
-    prev = g.stats()
-    for f in range(warmup + steps):
-        g.async_step(sc.dt)
-        g.wait_step_end()
-        ms = g.last_frame_ms()
-        st = g.stats()
-        if f >= warmup:
-            frame_ms.append(ms[0])
-            search_ms.append(ms[2])
-            found_d.append(st["found"] - prev["found"])
-            exp_d.append(st["expansions"] - prev["expansions"])
-        prev = st
-        # continuous re-path: agents whose search ended get a new target
-        xy, state = g.agents()
-        done = np.nonzero(state != G.SEARCH)[0]
-        if len(done):
-            tg = S.log_uniform_targets(sc.seed, 302, xy[done], 10.0, min(4000.0, W), W, W, sc.locate,
-                                       counters, ids=done)
-            g.set_agent_targets(done.astype(np.int32), tg)
-        if f == warmup - 1 or (f >= warmup and (f - warmup) % 30 == 29):
-            log(f"mode={mode} frame {f}: {ms[0]:.2f} ms, found so far {st['found']}")
-    final = g.stats()
-    info = g.launch_info()
-    g.close()
-    fm = np.array(frame_ms)
-    tot_s = fm.sum() / 1e3
-    return dict(
-        solves_per_s=float(np.sum(found_d) / tot_s) if tot_s > 0 else 0.0,
-        p50_ms=float(np.percentile(fm, 50)), p99_ms=float(np.percentile(fm, 99)),
-        mean_ms=float(fm.mean()), found=int(np.sum(found_d)), expansions=int(np.sum(exp_d)),
-        expansions_per_s=float(np.sum(exp_d) / tot_s) if tot_s > 0 else 0.0,
-        search_ms=float(np.mean(search_ms)) if profile else None,
-        stats=final, launch=info)
+def measured_traffic(T, Q, budget, n):
+    """HBM bytes per k_search launch from the committed rocprofv3 counter
+    summary, when it was taken with the same (T, Q, budget, agents)."""
+    path = os.path.join(ROOT, "profiles", "k_search_maze16k_pmc.md")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    if f"profile_config: T={T} Q={Q} budget={budget} agents={n}" not in txt:
+        return None
+    for line in txt.splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def roofline_search(sc, T, Q, budget, frames=6):
     """Search-kernel HBM roofline: algorithmic bytes 48*E + 16*D + 8*L per
-    launch (E expansions, D discoveries, L corridor nodes) over the kernel's
-    event-timed duration on its own stream (profiled batched frames)."""
-    cfg = G.Config(agents_max_count=sc.n, agent_radius=sc.radius, agent_speed=sc.speed,
-                   path_finder_parallel_agents=sc.P, path_finder_threads_per_agents=T,
-                   path_finder_queue_capacity=Q, path_finder_heuristic_k=sc.K,
-                   step_mode=G.MODE_BATCHED, round_budget=budget, is_profiler=True)
-    g = G.GMAPF()
-    g.init(cfg)
-    sc.build(g)
-    for x, y in sc.starts:
-        g.add_agent(x, y)
-    g.set_agent_targets(np.arange(sc.n, dtype=np.int32), sc.targets)
+    launch (E expansions, D discoveries, L corridor nodes; device counters)
+    over the kernel's HIP-event duration on its own stream (profiled batched
+    frames, same kernels as the captured graph)."""
+    g = make_world(sc, G.MODE_BATCHED, T, Q, budget, profile=True)
     counters = np.array(sc.counters, copy=True)
-    g.manual_start()
     prev = g.stats()
     tot_bytes, tot_ms = 0.0, 0.0
     for f in range(frames):
@@ -144,45 +87,23 @@ def roofline_search(sc, T, Q, budget, frames=6):
             tot_bytes += 48.0 * E + 16.0 * D + 8.0 * L
             tot_ms += ms[2]
         prev = st
-        xy, state = g.agents()
-        done = np.nonzero(state != G.SEARCH)[0]
-        if len(done):
-            tg = S.log_uniform_targets(sc.seed, 302, xy[done], 10.0, min(4000.0, sc.width), sc.width, sc.width,
-                                       sc.locate, counters, ids=done)
-            g.set_agent_targets(done.astype(np.int32), tg)
+        retarget(g, sc, counters)
     g.close()
+    n = max(1, frames - 2)
     gbs = tot_bytes / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else 0.0
-    return dict(bound="hbm", achieved=round(gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(gbs / HBM_PEAK_GBS, 5), traffic=measured_traffic(T, Q, budget, sc.n),
-                kernel="k_search (persistent)", bytes_per_launch=tot_bytes / max(1, frames - 2),
-                ms_per_launch=tot_ms / max(1, frames - 2))
+    return dict(bound="hbm", achieved=round(float(gbs), 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(float(gbs) / HBM_PEAK_GBS, 5), traffic=measured_traffic(T, Q, budget, sc.n),
+                bytes_per_launch=tot_bytes / n, ms_per_launch=tot_ms / n)
 
 
-def measured_traffic(T, Q, budget, n):
-    """HBM bytes per k_search launch from the committed rocprofv3 counter
-    summary, when it was taken with the same (T, Q, budget, agents)."""
-    path = os.path.join(ROOT, "profiles", "k_search_maze16k_pmc.md")
-    try:
-        txt = open(path).read()
-    except OSError:
-        return None
-    want = f"profile_config: T={T} Q={Q} budget={budget} agents={n}"
-    if want not in txt:
-        return None
-    for line in txt.splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
-    return None
-
-
-def cpu_baseline(sc, T, Q, seconds=12.0):
+def cpu_baseline(sc, loc, T, Q, seconds=12.0):
     """Single-core oracle (the CPU restatement) on a bounded sample of the
     same request stream: every agent's start node with its 1st, 2nd, ...
     log-uniform target, searched until ~`seconds` of CPU time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
-    g = make_locator.keep[-1]
+    g = loc.keep[-1]
     tri, nbr, _ = g.map_arrays()
     om = pyoracle.OracleMap(tri, nbr)
     S_nodes = g.locate(sc.starts)
@@ -200,8 +121,8 @@ def cpu_baseline(sc, T, Q, seconds=12.0):
                 found += int(ok)
                 n += 1
         rep += 1
-        targets = S.log_uniform_targets(sc.seed, 302, sc.starts, 10.0, min(4000.0, sc.width), sc.width, sc.width,
-                                        sc.locate, counters)
+        targets = S.log_uniform_targets(sc.seed, 302, sc.starts, 10.0, min(4000.0, sc.width), sc.width,
+                                        sc.height, sc.locate, counters)
     el = time.process_time() - t0
     return dict(value=round(found / el, 2), unit="solves/s", cores=1, kind="port",
                 sample=f"{n} searches ({rep} passes over the {sc.n} agents' start nodes with log-uniform targets) "
@@ -223,25 +144,24 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
-                                   "(see DESIGN.md), run with --gpus 1"}))
+                                   "(DESIGN.md section 8), run with --gpus 1"}))
         sys.exit(2)
     t_all = time.time()
-    sc = S.config3(make_locator, n_agents=args.agents, cells=args.cells)
-    tp = tuned_pair(sc.name, sc.n)
-    T, Q, B = (tp if tp else (8, 32, 192))
-    T = args.T or T
-    Q = args.Q or Q
-    B = args.budget or B
-    log(f"workload {sc.name}: {sc.n} agents, T={T} Q={Q} budget={B} (tuned={tp is not None})")
+    loc = Locators()
+    sc = S.config3(loc, n_agents=args.agents, cells=args.cells)
+    tp = tuned_triple(sc.name, sc.n)
+    T, Q, B = tp if tp else (8, 32, 192)
+    T, Q, B = args.T or T, args.Q or Q, args.budget or B
+    log(f"workload {sc.name}: {sc.n} agents, T={T} Q={Q} budget={B} (autotuned={tp is not None})")
     plain = None
     if not args.skip_plain:
-        plain = run_mode(sc, G.MODE_PLAIN, T, Q, 0, args.warmup, args.steps)
+        plain = run_frames(sc, G.MODE_PLAIN, T, Q, 0, args.warmup, args.steps, log=log)
         log("plain", {k: v for k, v in plain.items() if k not in ("stats", "launch")})
-    bat = run_mode(sc, G.MODE_BATCHED, T, Q, B, args.warmup, args.steps)
+    bat = run_frames(sc, G.MODE_BATCHED, T, Q, B, args.warmup, args.steps, log=log)
     log("batched", {k: v for k, v in bat.items() if k not in ("stats", "launch")})
     roof = roofline_search(sc, T, Q, B)
     log("roofline", roof)
-    cpu = None if args.skip_cpu else cpu_baseline(sc, T, Q)
+    cpu = None if args.skip_cpu else cpu_baseline(sc, loc, T, Q)
     out = {
         "metric": "path solves/s @16k agents (batched step), with p99 frame ms and plain-step ratio",
         "value": round(bat["solves_per_s"], 1),
@@ -259,13 +179,15 @@ def main():
             "workload": f"{sc.name} spanning-tree maze (64-unit cells, 16-wide corridors), {sc.n} agents, "
                         "log-uniform 10-4000 start-target distance, continuous re-path",
             "navmesh_nodes": bat["launch"]["V"],
-            "T": T, "Q": Q, "round_budget": B,
+            "T": T, "Q": Q, "round_budget": B, "autotuned": tp is not None,
             "batched_p50_ms": round(bat["p50_ms"], 4), "batched_p99_ms": round(bat["p99_ms"], 4),
             "batched_expansions_per_s": round(bat["expansions_per_s"], 1),
             "plain_solves_per_s": round(plain["solves_per_s"], 1) if plain else None,
             "plain_p50_ms": round(plain["p50_ms"], 4) if plain else None,
             "plain_p99_ms": round(plain["p99_ms"], 4) if plain else None,
-            "batched_over_plain": round(bat["solves_per_s"] / plain["solves_per_s"], 3) if plain and plain["solves_per_s"] else None,
+            "batched_over_plain": (round(bat["solves_per_s"] / plain["solves_per_s"], 3)
+                                   if plain and plain["solves_per_s"] else None),
+            "search_kernel_ms_per_launch": round(roof["ms_per_launch"], 4),
             "parallelism": "single GPU",
         },
         "roofline": {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},

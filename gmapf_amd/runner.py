@@ -1,0 +1,90 @@
+"""Frame-loop driver shared by bench.py and the autotuner: builds a
+scenario's map and agents through the public API, runs frames with
+continuous re-path (an agent whose search ended gets its next log-uniform
+target before the next frame) and reports solves/s and frame-time
+percentiles."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import GMAPF, MODE_BATCHED, MODE_PLAIN, SEARCH, Config
+from . import scenarios as S
+
+
+class Locators:
+    """Keeps host-only GMAPF objects alive that serve exact point location
+    for scenario generation (FillMap* + the host grid, no GPU)."""
+
+    def __init__(self):
+        self.keep = []
+
+    def __call__(self, W, H, obs):
+        g = GMAPF()
+        g.init(Config(agents_max_count=1))
+        g.fill_map_start(W, H)
+        for o in obs:
+            g.fill_map_obstacle(o)
+        g.fill_map_end()
+        self.keep.append(g)
+        return lambda p: g.locate(p)
+
+
+def make_world(sc, mode, T, Q, budget, profile=False, hash_cell=0.0):
+    cfg = Config(agents_max_count=sc.n, agent_radius=sc.radius, agent_speed=sc.speed,
+                 path_finder_parallel_agents=sc.P, path_finder_threads_per_agents=T,
+                 path_finder_queue_capacity=Q, path_finder_heuristic_k=sc.K, step_mode=mode,
+                 round_budget=max(1, budget), is_profiler=profile, hash_cell=hash_cell)
+    g = GMAPF()
+    g.init(cfg)
+    sc.build(g)
+    for x, y in sc.starts:
+        g.add_agent(x, y)
+    g.set_agent_targets(np.arange(sc.n, dtype=np.int32), sc.targets)
+    g.manual_start()
+    return g
+
+
+def retarget(g, sc, counters):
+    xy, state = g.agents()
+    done = np.nonzero(state != SEARCH)[0]
+    if len(done):
+        tg = S.log_uniform_targets(sc.seed, 302, xy[done], 10.0, min(4000.0, sc.width), sc.width, sc.height,
+                                   sc.locate, counters, ids=done)
+        g.set_agent_targets(done.astype(np.int32), tg)
+
+
+def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None):
+    """Continuous re-path frames; returns timing and counter statistics."""
+    g = make_world(sc, mode, T, Q, budget, profile)
+    counters = np.array(sc.counters, copy=True)
+    frame_ms, found_d, exp_d, search_ms = [], [], [], []
+    prev = g.stats()
+    for f in range(warmup + steps):
+        g.async_step(sc.dt)
+        g.wait_step_end()
+        ms = g.last_frame_ms()
+        st = g.stats()
+        if f >= warmup:
+            frame_ms.append(ms[0])
+            search_ms.append(ms[2])
+            found_d.append(st["found"] - prev["found"])
+            exp_d.append(st["expansions"] - prev["expansions"])
+        prev = st
+        retarget(g, sc, counters)
+        if log and (f == warmup - 1 or (f >= warmup and (f - warmup) % 30 == 29)):
+            log(f"mode={mode} frame {f}: {ms[0]:.2f} ms, found so far {st['found']}")
+    final = g.stats()
+    info = g.launch_info()
+    g.close()
+    fm = np.array(frame_ms)
+    tot_s = fm.sum() / 1e3
+    return dict(
+        solves_per_s=float(np.sum(found_d) / tot_s) if tot_s > 0 else 0.0,
+        p50_ms=float(np.percentile(fm, 50)), p99_ms=float(np.percentile(fm, 99)),
+        mean_ms=float(fm.mean()), found=int(np.sum(found_d)), expansions=int(np.sum(exp_d)),
+        expansions_per_s=float(np.sum(exp_d) / tot_s) if tot_s > 0 else 0.0,
+        search_ms=float(np.mean(search_ms)) if profile else None,
+        not_found=final["not_found"], stats=final, launch=info)
+
+
+__all__ = ["Locators", "make_world", "retarget", "run_frames", "MODE_PLAIN", "MODE_BATCHED"]
