@@ -327,7 +327,7 @@ class GMAPF:
         return dict(zip(keys, (int(x) for x in out)))
 
     def last_frame_ms(self):
-        out = np.zeros(5, dtype=np.float64)
+        out = np.zeros(8, dtype=np.float64)
         self._check(self._lib.gmapf_last_frame_ms(self._h, _ptr(out)))
         return out
 

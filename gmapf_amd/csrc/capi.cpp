@@ -79,18 +79,20 @@ struct gmapf_s {
     float* h_pos_stage = nullptr;
     int* h_state_stage = nullptr;
     unsigned long long* h_stats = nullptr;
-    int* h_upd = nullptr;       // ids | targ
-    float* h_upd_xy = nullptr;
+    int* h_upd = nullptr;       // [0] count, then ids[N], targ[N]
+    float2* h_upd_xy = nullptr;
     int upd_cap = 0;
-    int* d_upd = nullptr;
-    float2* d_upd_xy = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 
     // profiling
     Clock::time_point t_step;
-    double last_ms[5] = {0, 0, 0, 0, 0};
-    double sum_ms[5] = {0, 0, 0, 0, 0}, max_ms[5] = {0, 0, 0, 0, 0};
+    // [0] wall [1] admit [2] search [3] unused [4] crowd (device, profiler)
+    // [5] host enqueue [6] host wait for the device [7] host readback
+    double last_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double sum_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0}, max_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    Clock::time_point t_enq;
+    hipEvent_t ev_done = nullptr;
     long long frames = 0;
     long long stats_host[8] = {0};
 
@@ -134,6 +136,8 @@ struct gmapf_s {
             if (e) hipEventDestroy(e);
         if (ev_fork) hipEventDestroy(ev_fork);
         if (ev_join) hipEventDestroy(ev_join);
+        if (ev_done) hipEventDestroy(ev_done);
+        ev_done = nullptr;
         if (s1) hipStreamDestroy(s1);
         if (s2) hipStreamDestroy(s2);
         s1 = s2 = nullptr;
@@ -175,6 +179,8 @@ struct gmapf_s {
         if (started) throw ApiError(GMAPF_E_STATE, "already started");
         if (!map_filled) throw ApiError(GMAPF_E_STATE, "map is not filled");
         check_config();
+        // the search packs node ids into 23-bit fields of its claim hash
+        if (nav.V >= (1 << 23)) throw ApiError(GMAPF_E_ARG, "navmesh has more than 2^23 - 1 nodes");
         dev = cfg.device;
         HIPCHK(hipSetDevice(dev));
         started = true;  // from here release() frees what was allocated
@@ -196,6 +202,7 @@ struct gmapf_s {
         HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming));
         for (auto& e : ev_p) HIPCHK(hipEventCreate(&e));
 
         // ---- navmesh
@@ -362,10 +369,18 @@ struct gmapf_s {
         HIPCHK(hipHostMalloc((void**)&h_state_stage, Nn * sizeof(int)));
         HIPCHK(hipHostMalloc((void**)&h_stats, 8 * sizeof(unsigned long long)));
         upd_cap = std::max(N, 1);
-        HIPCHK(hipHostMalloc((void**)&h_upd, 2 * (size_t)upd_cap * sizeof(int)));
-        HIPCHK(hipHostMalloc((void**)&h_upd_xy, 2 * (size_t)upd_cap * sizeof(float)));
-        d_upd = dalloc<int>(2 * (size_t)upd_cap, "target updates");
-        d_upd_xy = dalloc<float2>(upd_cap, "target update positions");
+        HIPCHK(hipHostMalloc((void**)&h_upd, (1 + 2 * (size_t)upd_cap) * sizeof(int)));
+        HIPCHK(hipHostMalloc((void**)&h_upd_xy, (size_t)upd_cap * sizeof(float2)));
+        h_upd[0] = 0;
+        *h_step = 0.f;
+        dfr.h_step = h_step;
+        dfr.h_upd_n = h_upd;
+        dfr.h_upd_ids = h_upd + 1;
+        dfr.h_upd_targ = h_upd + 1 + upd_cap;
+        dfr.h_upd_xy = h_upd_xy;
+        dfr.h_pos_out = reinterpret_cast<float2*>(h_pos_stage);
+        dfr.h_state_out = h_state_stage;
+        dfr.h_stats_out = h_stats;
 
         // ---- search launch shape
         sp = SearchParams{T, Q, cfg.path_finder_heuristic_k, 0};
@@ -397,6 +412,7 @@ struct gmapf_s {
         HIPCHK(launch_crowd(dmap, dag, dfr, cp, s2));
         HIPCHK(hipEventRecord(ev_join, s2));
         HIPCHK(hipStreamWaitEvent(s1, ev_join, 0));
+        HIPCHK(launch_export(dag, dfr, s1));
     }
 
     void debug_sync(const char* phase) {
@@ -419,6 +435,7 @@ struct gmapf_s {
         HIPCHK(launch_crowd(dmap, dag, dfr, cp, s1));
         if (cfg.is_debug) debug_sync("crowd");
         HIPCHK(hipEventRecord(ev_p[4], s1));
+        HIPCHK(launch_export(dag, dfr, s1));
     }
 
     void async_step(float dt) {
@@ -427,20 +444,19 @@ struct gmapf_s {
         if (!started) manual_start();
         HIPCHK(hipSetDevice(dev));
         t_step = Clock::now();
+        // frame inputs go to pinned host memory that the frame's first
+        // kernels read directly (no copies; the captured graph stays fixed)
         *h_step = cfg.agent_speed * dt;  // AgentsMover.h:299
-        HIPCHK(hipMemcpyAsync(dfr.stepv, h_step, sizeof(float), hipMemcpyHostToDevice, s1));
+        h_upd[0] = 0;
         if (!upd_ids.empty()) {
             compact_updates();  // one update per agent: the last one wins
-            int n = (int)upd_ids.size();
+            const int n = (int)upd_ids.size();
             for (int k = 0; k < n; k++) {
-                h_upd[k] = upd_ids[k];
-                h_upd[n + k] = upd_targ[k];
-                h_upd_xy[2 * k] = upd_xy[2 * k];
-                h_upd_xy[2 * k + 1] = upd_xy[2 * k + 1];
+                h_upd[1 + k] = upd_ids[k];
+                h_upd[1 + upd_cap + k] = upd_targ[k];
+                h_upd_xy[k] = make_float2(upd_xy[2 * k], upd_xy[2 * k + 1]);
             }
-            HIPCHK(hipMemcpyAsync(d_upd, h_upd, 2 * (size_t)n * sizeof(int), hipMemcpyHostToDevice, s1));
-            HIPCHK(hipMemcpyAsync(d_upd_xy, h_upd_xy, (size_t)n * sizeof(float2), hipMemcpyHostToDevice, s1));
-            HIPCHK(launch_apply_updates(dag, d_upd, d_upd + n, d_upd_xy, n, s1));
+            h_upd[0] = n;
             upd_ids.clear();
             upd_targ.clear();
             upd_xy.clear();
@@ -465,10 +481,8 @@ struct gmapf_s {
         } else {
             enqueue_frame();
         }
-        const int N = std::max(n_agents, 1);
-        HIPCHK(hipMemcpyAsync(h_pos_stage, dag.pos, N * sizeof(float2), hipMemcpyDeviceToHost, s1));
-        HIPCHK(hipMemcpyAsync(h_state_stage, dag.state, N * sizeof(int), hipMemcpyDeviceToHost, s1));
-        HIPCHK(hipMemcpyAsync(h_stats, dfr.stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s1));
+        HIPCHK(hipEventRecord(ev_done, s1));
+        t_enq = Clock::now();
         step_going = true;
     }
 
@@ -477,8 +491,14 @@ struct gmapf_s {
         if (!step_going) return;
         HIPCHK(hipSetDevice(dev));
         step_going = false;
-        HIPCHK(hipStreamSynchronize(s1));
+        // poll the frame's completion event: lower wake-up latency than a
+        // blocking stream synchronisation, like CUDA's default spin schedule
+        hipError_t q;
+        while ((q = hipEventQuery(ev_done)) == hipErrorNotReady) {
+        }
+        HIPCHK(q);
         HIPCHK(hipGetLastError());
+        const Clock::time_point t_sync = Clock::now();
         if (n_agents > 0) {
             std::memcpy(pos.data(), h_pos_stage, (size_t)n_agents * sizeof(float2));
             std::memcpy(state.data(), h_state_stage, (size_t)n_agents * sizeof(int));
@@ -488,7 +508,11 @@ struct gmapf_s {
         stats_host[7] = frames;
         // target changes made while the step ran stay pending for the next step
         for (int id : upd_ids) state[id] = GMAPF_STATE_SEARCH;
-        last_ms[0] = std::chrono::duration<double, std::milli>(Clock::now() - t_step).count();
+        const Clock::time_point t_end = Clock::now();
+        last_ms[0] = std::chrono::duration<double, std::milli>(t_end - t_step).count();
+        last_ms[5] = std::chrono::duration<double, std::milli>(t_enq - t_step).count();
+        last_ms[6] = std::chrono::duration<double, std::milli>(t_sync - t_enq).count();
+        last_ms[7] = std::chrono::duration<double, std::milli>(t_end - t_sync).count();
         if (cfg.is_profiler || cfg.is_debug) {
             for (int k = 0; k < 4; k++) {
                 float ms = 0;
@@ -496,7 +520,7 @@ struct gmapf_s {
                 last_ms[k + 1] = ms;
             }
         }
-        for (int k = 0; k < 5; k++) {
+        for (int k = 0; k < 8; k++) {
             sum_ms[k] += last_ms[k];
             max_ms[k] = std::max(max_ms[k], last_ms[k]);
         }
@@ -911,10 +935,10 @@ int gmapf_get_stats(gmapf_handle h, long long* out8) {
     });
 }
 
-int gmapf_last_frame_ms(gmapf_handle h, double* out5) {
+int gmapf_last_frame_ms(gmapf_handle h, double* out8) {
     return guarded(h, [&] {
         h->require_init();
-        for (int k = 0; k < 5; k++) out5[k] = h->last_ms[k];
+        for (int k = 0; k < 8; k++) out8[k] = h->last_ms[k];
     });
 }
 

@@ -109,7 +109,17 @@ struct DevFrame {
     int* horder;      // N
     int* hblk;        // scan partials over cells
     int ncell;
-    float* stepv;     // speed * dt of the frame (written by the host before launch)
+    float* stepv;                // device copy of speed * dt, made by the admission kernel
+    // pinned host memory read / written by the frame's kernels (no copies,
+    // so a captured frame is one graph launch)
+    const float* h_step;         // speed * dt of the frame
+    const int* h_upd_n;          // number of target updates
+    const int* h_upd_ids;        // [n] agent ids (at most one per agent)
+    const int* h_upd_targ;       // [n] target nodes
+    const float2* h_upd_xy;      // [n] target positions
+    float2* h_pos_out;           // [N] positions after the frame
+    int* h_state_out;            // [N] states after the frame
+    unsigned long long* h_stats_out;  // [8] cumulative counters
 };
 
 struct SearchParams {

@@ -163,69 +163,180 @@ __device__ int locate(const DevMap& m, float x, float y) {
 }
 
 // ------------------------------------------------------------- admission
-__global__ void k_cancel(DevAgents ag, DevSlots sl, DevFrame fr) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= sl.P) return;
-    int a = sl.agent[s];
-    if (a >= 0 && ag.gen[a] != sl.gen[s]) {  // target changed while in flight
-        sl.agent[s] = -1;
-        ag.inflight[a] = -1;
-        a = -1;
+constexpr int kAdmitThreads = 1024;
+constexpr int kAdmitWaves = kAdmitThreads / 64;
+
+// exclusive scan of one value per thread over the 1024-thread block; returns
+// the thread's prefix and the block total.  Starts with a barrier, so
+// consecutive calls may reuse sh.
+__device__ __forceinline__ int admit_scan(int v, int* sh, int* total) {
+    const int ln = threadIdx.x % 64, w = threadIdx.x / 64;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (ln >= o) x += y;
     }
-    fr.sflags[s] = (a < 0) ? 1 : 0;
+    __syncthreads();
+    if (ln == 63) sh[w] = x;
+    __syncthreads();
+    int wpre = 0, tot = 0;
+    for (int k = 0; k < kAdmitWaves; ++k) {
+        const int t = sh[k];
+        wpre += (k < w) ? t : 0;
+        tot += t;
+    }
+    *total = tot;
+    return wpre + x - v;
 }
 
-// PathFinder.h:85-96 + AgentsMover.h:39-46
-__global__ void k_agent_flags(DevAgents ag, DevFrame fr) {
-    __shared__ int sh[4];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int f = 0, mv = 0;
-    if (i < ag.n) {
-        int st = ag.state[i];
-        mv = (st == kMove) ? 1 : 0;
-        ag.in_m[i] = mv;
-        if (st == kSearch && ag.inflight[i] < 0) {
-            if (ag.curr[i] == ag.targ[i])
-                ag.state[i] = kIdle;
-            else
-                f = 1;
+// F1 (PathFinder.h:85-116) and F2 (AgentsMover.h:39-46) in one workgroup:
+//   1. in-flight searches whose agent got a new target are dropped;
+//   2. free slots listed in ascending slot index;
+//   3. requests in ascending agent index = Search agents not in flight; a
+//      start node equal to the target node makes the agent Idle
+//      (PathFinder.h:91-95); the r-th request takes the r-th free slot, the
+//      rest stay Search (PathFinder.h:97-102); the moving set = agents in Move;
+//   4. active list = occupied slots in ascending slot index, ticket reset.
+// Each thread owns kAdmitItems consecutive indices of a chunk, so one block
+// scan covers kAdmitThreads * kAdmitItems indices.
+constexpr int kAdmitItems = 8;
+constexpr int kAdmitChunk = kAdmitThreads * kAdmitItems;
+
+__global__ __launch_bounds__(kAdmitThreads) void k_admit_fused(DevAgents ag, DevSlots sl, DevFrame fr) {
+    __shared__ int sh[kAdmitWaves];
+    __shared__ int nmove_sh;
+    const int tid = threadIdx.x;
+    const int P = sl.P, N = ag.n;
+    if (tid == 0) {
+        *fr.stepv = *fr.h_step;
+        nmove_sh = 0;
+    }
+    for (int s = tid; s < P; s += kAdmitThreads) {
+        const int a = sl.agent[s];
+        if (a >= 0 && ag.gen[a] != sl.gen[s]) {
+            sl.agent[s] = -1;
+            ag.inflight[a] = -1;
         }
-        fr.flags[i] = f;
     }
-    int t = block_reduce_sum(mv, sh);
-    if (threadIdx.x == 0 && t) atomicAdd(&fr.counters[5], t);
+    __syncthreads();
+    int nfree = 0;
+    for (int base = 0; base < P; base += kAdmitChunk) {
+        const int s0 = base + tid * kAdmitItems;
+        int f[kAdmitItems];
+        int c = 0;
+#pragma unroll
+        for (int j = 0; j < kAdmitItems; ++j) {
+            f[j] = (s0 + j < P && sl.agent[s0 + j] < 0) ? 1 : 0;
+            c += f[j];
+        }
+        int tot;
+        int r = nfree + admit_scan(c, sh, &tot);
+#pragma unroll
+        for (int j = 0; j < kAdmitItems; ++j)
+            if (f[j]) fr.free_list[r++] = s0 + j;
+        nfree += tot;
+    }
+    int nreq = 0, mv = 0;
+    for (int base = 0; base < N; base += kAdmitChunk) {
+        const int i0 = base + tid * kAdmitItems;
+        int f[kAdmitItems];
+        int c = 0;
+#pragma unroll
+        for (int j = 0; j < kAdmitItems; ++j) {
+            const int i = i0 + j;
+            f[j] = 0;
+            if (i < N) {
+                const int st = ag.state[i];
+                ag.in_m[i] = (st == kMove) ? 1 : 0;
+                mv += (st == kMove) ? 1 : 0;
+                if (st == kSearch && ag.inflight[i] < 0) {
+                    if (ag.curr[i] == ag.targ[i])
+                        ag.state[i] = kIdle;
+                    else
+                        f[j] = 1;
+                }
+            }
+            c += f[j];
+        }
+        int tot;
+        int r = nreq + admit_scan(c, sh, &tot);
+#pragma unroll
+        for (int j = 0; j < kAdmitItems; ++j) {
+            if (!f[j]) continue;
+            if (r < nfree) {
+                const int i = i0 + j;
+                const int s = fr.free_list[r];
+                sl.agent[s] = i;
+                sl.gen[s] = ag.gen[i];
+                sl.S[s] = ag.curr[i];
+                sl.Tg[s] = ag.targ[i];
+                sl.stamp[s] = sl.stamp[s] + 1u;
+                sl.status[s] = 3;
+                sl.rounds[s] = 0;
+                sl.expansions[s] = 0;
+                sl.discoveries[s] = 0;
+                ag.inflight[i] = s;
+                ag.path_len[i] = 0;  // old corridor released (PathFinder.h:104-108)
+            }
+            ++r;
+        }
+        nreq += tot;
+    }
+    if (mv) atomicAdd(&nmove_sh, mv);
+    __syncthreads();
+    int nact = 0;
+    for (int base = 0; base < P; base += kAdmitChunk) {
+        const int s0 = base + tid * kAdmitItems;
+        int f[kAdmitItems];
+        int c = 0;
+#pragma unroll
+        for (int j = 0; j < kAdmitItems; ++j) {
+            f[j] = (s0 + j < P && sl.agent[s0 + j] >= 0) ? 1 : 0;
+            c += f[j];
+        }
+        int tot;
+        int r = nact + admit_scan(c, sh, &tot);
+#pragma unroll
+        for (int j = 0; j < kAdmitItems; ++j)
+            if (f[j]) fr.active[r++] = s0 + j;
+        nact += tot;
+    }
+    if (tid == 0) {
+        const int admitted = nreq < nfree ? nreq : nfree;
+        fr.counters[0] = nreq;
+        fr.counters[1] = nact;
+        fr.counters[2] = 0;
+        fr.counters[3] = 0;
+        fr.counters[4] = nfree;
+        fr.counters[5] = nmove_sh;
+        fr.counters[6] = admitted;
+        fr.stats[6] += (unsigned long long)admitted;
+    }
 }
 
-__global__ void k_compact(const int* flags, const int* ranks, int n, int* out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && flags[i]) out[ranks[i]] = i;
+// GMAPF.cu:247-250 for target changes made after start, read from pinned
+// host memory; the host passes at most one update per agent (the last one).
+__global__ void k_apply_updates(DevAgents ag, DevFrame fr) {
+    __shared__ int n_sh;
+    if (threadIdx.x == 0) n_sh = *fr.h_upd_n;
+    __syncthreads();
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_sh) return;
+    const int i = fr.h_upd_ids[k];
+    ag.state[i] = kSearch;
+    ag.targ[i] = fr.h_upd_targ[k];
+    ag.targ_pos[i] = fr.h_upd_xy[k];
+    ag.gen[i] = ag.gen[i] + 1;
 }
 
-// r-th request (ascending agent index) takes the r-th free slot (ascending slot index)
-__global__ void k_admit(DevAgents ag, DevSlots sl, DevFrame fr) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ag.n || !fr.flags[i]) return;
-    const int r = fr.ranks[i];
-    if (r >= fr.counters[4]) return;  // no free slot: stays Search (PathFinder.h:97-102)
-    const int s = fr.free_list[r];
-    sl.agent[s] = i;
-    sl.gen[s] = ag.gen[i];
-    sl.S[s] = ag.curr[i];
-    sl.Tg[s] = ag.targ[i];
-    sl.stamp[s] = sl.stamp[s] + 1u;
-    sl.status[s] = 3;
-    sl.rounds[s] = 0;
-    sl.expansions[s] = 0;
-    sl.discoveries[s] = 0;
-    ag.inflight[i] = s;
-    ag.path_len[i] = 0;  // old corridor released (PathFinder.h:104-108)
-    atomicAdd(&fr.stats[6], 1ull);
-}
-
-__global__ void k_slot_active(DevSlots sl, DevFrame fr) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < sl.P) fr.sflags[s] = (sl.agent[s] >= 0) ? 1 : 0;
-    if (s == 0) fr.counters[2] = 0;
+// positions, states and counters of the finished frame into pinned host memory
+__global__ void k_export(DevAgents ag, DevFrame fr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ag.n) {
+        fr.h_pos_out[i] = ag.pos[i];
+        fr.h_state_out[i] = ag.state[i];
+    }
+    if (i < 8) fr.h_stats_out[i] = fr.stats[i];
 }
 
 // ------------------------------------------------------------- crowd
@@ -373,18 +484,6 @@ __global__ void k_progress(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp
     }
 }
 
-// GMAPF.cu:247-250 for target changes made after start; the host passes at
-// most one update per agent (the last one), so the updates are independent.
-__global__ void k_apply_updates(DevAgents ag, const int* ids, const int* targ, const float2* xy, int n) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int i = ids[k];
-    ag.state[i] = kSearch;
-    ag.targ[i] = targ[k];
-    ag.targ_pos[i] = xy[k];
-    ag.gen[i] = ag.gen[i] + 1;
-}
-
 __global__ void k_locate(DevMap map, const float2* pts, int n, int* out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = locate(map, pts[i].x, pts[i].y);
@@ -409,19 +508,13 @@ hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, long lon
 hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& sl,
                         const DevFrame& fr, hipStream_t st) {
     (void)map;
-    hipError_t e;
-    if ((e = launch_fill(fr.counters, 0, 8, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cancel, dim3(nblk(sl.P, 256)), dim3(256), 0, st, ag, sl, fr);
-    hipLaunchKernelGGL(k_agent_flags, dim3(nblk(ag.n, 256)), dim3(256), 0, st, ag, fr);
-    if ((e = exscan(fr.flags, ag.n, fr.blk, fr.ranks, &fr.counters[0], st)) != hipSuccess) return e;
-    if ((e = exscan(fr.sflags, sl.P, fr.sblk, fr.sranks, &fr.counters[4], st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_compact, dim3(nblk(sl.P, 256)), dim3(256), 0, st, (const int*)fr.sflags,
-                       (const int*)fr.sranks, sl.P, fr.free_list);
-    hipLaunchKernelGGL(k_admit, dim3(nblk(ag.n, 256)), dim3(256), 0, st, ag, sl, fr);
-    hipLaunchKernelGGL(k_slot_active, dim3(nblk(sl.P, 256)), dim3(256), 0, st, sl, fr);
-    if ((e = exscan(fr.sflags, sl.P, fr.sblk, fr.sranks, &fr.counters[1], st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_compact, dim3(nblk(sl.P, 256)), dim3(256), 0, st, (const int*)fr.sflags,
-                       (const int*)fr.sranks, sl.P, fr.active);
+    hipLaunchKernelGGL(k_apply_updates, dim3(nblk(ag.n > 0 ? ag.n : 1, 256)), dim3(256), 0, st, ag, fr);
+    hipLaunchKernelGGL(k_admit_fused, dim3(1), dim3(kAdmitThreads), 0, st, ag, sl, fr);
+    return hipGetLastError();
+}
+
+hipError_t launch_export(const DevAgents& ag, const DevFrame& fr, hipStream_t st) {
+    hipLaunchKernelGGL(k_export, dim3(nblk(ag.n > 8 ? ag.n : 8, 256)), dim3(256), 0, st, ag, fr);
     return hipGetLastError();
 }
 
@@ -439,13 +532,6 @@ hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& 
     hipLaunchKernelGGL(k_hash_fixup, dim3(nb), dim3(256), 0, st, ag, fr);
     hipLaunchKernelGGL(k_avoid, dim3(nb), dim3(256), 0, st, map, ag, fr, cp);
     hipLaunchKernelGGL(k_progress, dim3(nb), dim3(256), 0, st, map, ag, fr, cp);
-    return hipGetLastError();
-}
-
-hipError_t launch_apply_updates(const DevAgents& ag, const int* ids, const int* targ,
-                                const float2* xy, int n, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_apply_updates, dim3(nblk(n, 256)), dim3(256), 0, st, ag, ids, targ, xy, n);
     return hipGetLastError();
 }
 

@@ -78,10 +78,19 @@ __device__ __forceinline__ void heap_push(unsigned long long* q, int L, int& cnt
         ++cnt;
         return;
     }
-    // full: the maximum is a leaf (indices Q/2 .. Q-1)
+    // full: the maximum is a leaf (indices Q/2 .. Q-1); read 4 leaves per step
     int mi = Q >> 1;
     unsigned long long mv = q[mi * kWave + L];
-    for (int i = mi + 1; i < Q; ++i) {
+    int i = mi + 1;
+    for (; i + 4 <= Q; i += 4) {
+        const unsigned long long v0 = q[i * kWave + L], v1 = q[(i + 1) * kWave + L];
+        const unsigned long long v2 = q[(i + 2) * kWave + L], v3 = q[(i + 3) * kWave + L];
+        if (v0 > mv) { mv = v0; mi = i; }
+        if (v1 > mv) { mv = v1; mi = i + 1; }
+        if (v2 > mv) { mv = v2; mi = i + 2; }
+        if (v3 > mv) { mv = v3; mi = i + 3; }
+    }
+    for (; i < Q; ++i) {
         const unsigned long long v = q[i * kWave + L];
         if (v > mv) {
             mv = v;
@@ -133,14 +142,31 @@ constexpr int cand_stride() {
     return ((3 * T + 3) / 4) * 4;
 }
 
+// per-segment claim hash: power of two >= 6T slots of (node << 8 | k)
+template <int T>
+constexpr int hash_bits() {
+    int b = 1;
+    while ((1 << b) < 6 * T) ++b;
+    return b;
+}
+constexpr int kHashEmpty = 0x7fffffff;
+
+__device__ __forceinline__ unsigned claim_hash(int n, int bits) {
+    return ((unsigned)n * 2654435761u) >> (32 - bits);
+}
+
 template <int T>
 __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlots slots, DevFrame fr,
                                                SearchParams sp, int persistent) {
     extern __shared__ unsigned long long q[];  // 64 * Q keys
     constexpr int RPW = kWave / T;
     constexpr int CS = cand_stride<T>();
+    constexpr int HB = hash_bits<T>();
+    constexpr int HS = 1 << HB;
     __shared__ __align__(16) int cand[RPW * CS];
     __shared__ unsigned long long stage[kWave * 3];
+    __shared__ int htab[RPW * HS];
+    for (int i = threadIdx.x; i < RPW * HS; i += kWave) htab[i] = kHashEmpty;
 
     const int L = threadIdx.x;
     const int sub = L / T;
@@ -250,27 +276,39 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 pdir[s] = (int)(vn.y & 1u);
             }
         }
+        int* hseg = &htab[sub * HS];
+        unsigned hslot[3] = {0u, 0u, 0u};
 #pragma unroll
-        for (int s = 0; s < 3; ++s) cand[cbase + l * 3 + s] = nn[s] >= 0 ? (nn[s] * 2 + cdir) : -1;
+        for (int s = 0; s < 3; ++s) {
+            cand[cbase + l * 3 + s] = nn[s] >= 0 ? (nn[s] * 2 + cdir) : -1;
+            if (nn[s] >= 0 && !prior[s]) {  // only unvisited nodes can be claimed
+                hslot[s] = claim_hash(nn[s], HB);
+                atomicMin(&hseg[hslot[s]], (nn[s] << 8) | (l * 3 + s));
+            }
+        }
         if (l == T - 1 && CS > 3 * T) {
             for (int k = 3 * T; k < CS; ++k) cand[cbase + k] = -1;
         }
         __syncthreads();
 
-        // ---- duplicate claims of this round: first earlier candidate with the same node
+        // ---- duplicate claims of this round: first earlier candidate with the
+        // same node, from the hash; a slot taken by another node falls back to
+        // scanning the earlier candidates
         int first[3] = {-1, -1, -1};
-        if (popped) {
-            const int4* c4 = reinterpret_cast<const int4*>(&cand[cbase]);
-            const int nchunk = (3 * l + 2 + 3) >> 2;
-            for (int ch = 0; ch < nchunk; ++ch) {
-                const int4 v = c4[ch];
-                const int vv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int k = ch * 4 + e;
-#pragma unroll
-                    for (int s = 0; s < 3; ++s) {
-                        if (first[s] < 0 && k < 3 * l + s && vv[e] >= 0 && (vv[e] >> 1) == nn[s]) first[s] = vv[e];
+        for (int s = 0; s < 3; ++s) {
+            if (nn[s] < 0 || prior[s]) continue;
+            const int e = hseg[hslot[s]];
+            const int k = l * 3 + s;
+            if ((e >> 8) == nn[s]) {
+                const int fk = e & 0xff;
+                if (fk < k) first[s] = cand[cbase + fk];
+            } else {
+                for (int kk = 0; kk < k; ++kk) {
+                    const int cv = cand[cbase + kk];
+                    if (cv >= 0 && (cv >> 1) == nn[s]) {
+                        first[s] = cv;
+                        break;
                     }
                 }
             }
@@ -321,6 +359,10 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
         }
         const unsigned long long pb = __ballot(popped) & smask;
         __syncthreads();
+        // empty this round's claim-hash slots for the next round
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+            if (nn[s] >= 0 && !prior[s]) hseg[hslot[s]] = kHashEmpty;
 
         // ---- deal frontier item f to lane f mod T
         if (has_req) {

@@ -20,18 +20,17 @@ hipError_t search_prepare(int T, int Q);
 hipError_t launch_fill(int* p, int v, long long n, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, long long n, hipStream_t st);
 
-// F1: cancel stale searches, mark requests / moving set, admit in ascending
-// agent index into free slots in ascending slot index, build the active list.
+// F0 + F1 + F2: target updates from pinned host memory, then cancel stale
+// searches, admit requests in ascending agent index into free slots in
+// ascending slot index, moving set, active list (one workgroup).
 hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& slots,
                         const DevFrame& fr, hipStream_t st);
+// positions, states and counters into pinned host memory
+hipError_t launch_export(const DevAgents& ag, const DevFrame& fr, hipStream_t st);
 // F4: move, hash + avoidance (Jacobi), progress; all no-ops when the moving
 // set of the frame is empty.
 hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& fr,
                         const CrowdParams& cp, hipStream_t st);
-
-// target changes made by the host after start (state Search, new target, gen+1)
-hipError_t launch_apply_updates(const DevAgents& ag, const int* ids, const int* targ,
-                                const float2* xy, int n, hipStream_t st);
 
 // bulk point location (min containing triangle or -1)
 hipError_t launch_locate(const DevMap& map, const float2* pts, int n, int* out, hipStream_t st);

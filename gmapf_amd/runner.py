@@ -57,7 +57,7 @@ def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None):
     """Continuous re-path frames; returns timing and counter statistics."""
     g = make_world(sc, mode, T, Q, budget, profile)
     counters = np.array(sc.counters, copy=True)
-    frame_ms, found_d, exp_d, search_ms = [], [], [], []
+    frame_ms, found_d, exp_d, search_ms, host_ms = [], [], [], [], []
     prev = g.stats()
     for f in range(warmup + steps):
         g.async_step(sc.dt)
@@ -67,6 +67,7 @@ def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None):
         if f >= warmup:
             frame_ms.append(ms[0])
             search_ms.append(ms[2])
+            host_ms.append(ms[5:8])
             found_d.append(st["found"] - prev["found"])
             exp_d.append(st["expansions"] - prev["expansions"])
         prev = st
@@ -84,6 +85,7 @@ def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None):
         mean_ms=float(fm.mean()), found=int(np.sum(found_d)), expansions=int(np.sum(exp_d)),
         expansions_per_s=float(np.sum(exp_d) / tot_s) if tot_s > 0 else 0.0,
         search_ms=float(np.mean(search_ms)) if profile else None,
+        host_enqueue_wait_readback_ms=[round(float(x), 4) for x in np.mean(np.array(host_ms), axis=0)],
         not_found=final["not_found"], stats=final, launch=info)
 
 

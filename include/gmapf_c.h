@@ -110,9 +110,10 @@ int gmapf_locate_device(gmapf_handle h, int n, const float* xy, int* out); /* sa
 /* cumulative device counters: [0] found [1] not found [2] expansions
  * [3] discoveries [4] corridor nodes [5] rounds [6] admitted [7] frames */
 int gmapf_get_stats(gmapf_handle h, long long* out8);
-/* last frame: host wall ms (AsyncStep entry -> WaitStepEnd return) and, when
- * profiling, per-phase device ms: [0] wall [1] admit [2] search [3] finish [4] crowd */
-int gmapf_last_frame_ms(gmapf_handle h, double* out5);
+/* last frame, ms: [0] host wall (AsyncStep entry -> WaitStepEnd return);
+ * with profiling, device phases [1] admit [2] search [4] crowd ([3] unused);
+ * host parts [5] enqueue [6] wait for the device [7] readback */
+int gmapf_last_frame_ms(gmapf_handle h, double* out8);
 /* spatial hash of the last frame (keys per agent, stable cell order) */
 int gmapf_get_hash(gmapf_handle h, int* keys, int* order, int* n);
 /* resolved build knobs: [0] T [1] Q [2] P [3] budget [4] search grid [5] mode [6] V */
