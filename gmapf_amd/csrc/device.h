@@ -62,6 +62,7 @@ struct DevAgents {
     long long path_cap;
     int* gen;
     int* inflight;
+    int* inflight_gen;  // gen of the agent when its in-flight search was admitted
     int* in_m;
 };
 
@@ -94,6 +95,7 @@ struct DevFrame {
     int* blk;        // block partials (agents)
     int* sblk;       // block partials (slots)
     int* free_list;  // P
+    int* adm_agent;  // P: agent of the k-th admission this frame
     int* active;     // P: active slot list for the search kernel
     int* ended;      // P: slots that ended this frame
     int* counters;   // [0] n_requests [1] n_active [2] work ticket [3] n_ended

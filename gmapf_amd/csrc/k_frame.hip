@@ -189,20 +189,23 @@ __device__ __forceinline__ int admit_scan(int v, int* sh, int* total) {
     return wpre + x - v;
 }
 
-// F1 (PathFinder.h:85-116) and F2 (AgentsMover.h:39-46) in one workgroup:
-//   1. in-flight searches whose agent got a new target are dropped;
-//   2. free slots listed in ascending slot index;
+// F1 (PathFinder.h:85-116) and F2 (AgentsMover.h:39-46), list building in one
+// workgroup with coalesced accesses only:
+//   1. agents whose target changed while their search was in flight drop it;
+//   2. occupied slots (ascending) start the active list, free slots are
+//      listed in ascending slot index;
 //   3. requests in ascending agent index = Search agents not in flight; a
 //      start node equal to the target node makes the agent Idle
 //      (PathFinder.h:91-95); the r-th request takes the r-th free slot, the
 //      rest stay Search (PathFinder.h:97-102); the moving set = agents in Move;
-//   4. active list = occupied slots in ascending slot index, ticket reset.
-// Each thread owns kAdmitItems consecutive indices of a chunk, so one block
-// scan covers kAdmitThreads * kAdmitItems indices.
+//   4. the admitted slots complete the active list.
+// The scattered per-slot initialisation of the admitted searches is done by
+// k_admit_slots across the whole GPU.  Each thread owns kAdmitItems
+// consecutive indices of a chunk; all loads of a chunk are issued first.
 constexpr int kAdmitItems = 8;
 constexpr int kAdmitChunk = kAdmitThreads * kAdmitItems;
 
-__global__ __launch_bounds__(kAdmitThreads) void k_admit_fused(DevAgents ag, DevSlots sl, DevFrame fr) {
+__global__ __launch_bounds__(kAdmitThreads) void k_admit_lists(DevAgents ag, DevSlots sl, DevFrame fr) {
     __shared__ int sh[kAdmitWaves];
     __shared__ int nmove_sh;
     const int tid = threadIdx.x;
@@ -211,50 +214,80 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_fused(DevAgents ag, Dev
         *fr.stepv = *fr.h_step;
         nmove_sh = 0;
     }
-    for (int s = tid; s < P; s += kAdmitThreads) {
-        const int a = sl.agent[s];
-        if (a >= 0 && ag.gen[a] != sl.gen[s]) {
-            sl.agent[s] = -1;
-            ag.inflight[a] = -1;
+    // 1
+    for (int base = 0; base < N; base += kAdmitChunk) {
+        const int i0 = base + tid * kAdmitItems;
+        int inf[kAdmitItems], g[kAdmitItems], ig[kAdmitItems];
+#pragma unroll
+        for (int j = 0; j < kAdmitItems; ++j) {
+            const bool ok = i0 + j < N;
+            inf[j] = ok ? ag.inflight[i0 + j] : -1;
+            g[j] = ok ? ag.gen[i0 + j] : 0;
+            ig[j] = ok ? ag.inflight_gen[i0 + j] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kAdmitItems; ++j) {
+            if (inf[j] >= 0 && g[j] != ig[j]) {
+                sl.agent[inf[j]] = -1;
+                ag.inflight[i0 + j] = -1;
+            }
         }
     }
     __syncthreads();
-    int nfree = 0;
+    // 2
+    int nfree = 0, nocc = 0;
     for (int base = 0; base < P; base += kAdmitChunk) {
         const int s0 = base + tid * kAdmitItems;
         int f[kAdmitItems];
         int c = 0;
 #pragma unroll
         for (int j = 0; j < kAdmitItems; ++j) {
-            f[j] = (s0 + j < P && sl.agent[s0 + j] < 0) ? 1 : 0;
-            c += f[j];
+            f[j] = (s0 + j < P) ? (sl.agent[s0 + j] < 0 ? 1 : 2) : 0;
+            c += (f[j] == 1) ? 1 : 0;
         }
         int tot;
         int r = nfree + admit_scan(c, sh, &tot);
+        // occupied slots before this thread's items = valid items before it - free ones
+        int o = nocc + tid * kAdmitItems - (r - nfree);
 #pragma unroll
-        for (int j = 0; j < kAdmitItems; ++j)
-            if (f[j]) fr.free_list[r++] = s0 + j;
+        for (int j = 0; j < kAdmitItems; ++j) {
+            if (f[j] == 1)
+                fr.free_list[r++] = s0 + j;
+            else if (f[j] == 2)
+                fr.active[o++] = s0 + j;
+        }
+        const int chunk_items = (P - base) < kAdmitChunk ? (P - base) : kAdmitChunk;
         nfree += tot;
+        nocc += chunk_items - tot;
     }
+    __syncthreads();  // free list visible to all threads
+    // 3
     int nreq = 0, mv = 0;
     for (int base = 0; base < N; base += kAdmitChunk) {
         const int i0 = base + tid * kAdmitItems;
+        int st[kAdmitItems], inf[kAdmitItems], cu[kAdmitItems], tg[kAdmitItems];
+#pragma unroll
+        for (int j = 0; j < kAdmitItems; ++j) {
+            const bool ok = i0 + j < N;
+            st[j] = ok ? ag.state[i0 + j] : kIdle;
+            inf[j] = ok ? ag.inflight[i0 + j] : 0;
+            cu[j] = ok ? ag.curr[i0 + j] : 0;
+            tg[j] = ok ? ag.targ[i0 + j] : 0;
+        }
         int f[kAdmitItems];
         int c = 0;
 #pragma unroll
         for (int j = 0; j < kAdmitItems; ++j) {
             const int i = i0 + j;
             f[j] = 0;
-            if (i < N) {
-                const int st = ag.state[i];
-                ag.in_m[i] = (st == kMove) ? 1 : 0;
-                mv += (st == kMove) ? 1 : 0;
-                if (st == kSearch && ag.inflight[i] < 0) {
-                    if (ag.curr[i] == ag.targ[i])
-                        ag.state[i] = kIdle;
-                    else
-                        f[j] = 1;
-                }
+            if (i >= N) continue;
+            ag.in_m[i] = (st[j] == kMove) ? 1 : 0;
+            mv += (st[j] == kMove) ? 1 : 0;
+            if (st[j] == kSearch && inf[j] < 0) {
+                if (cu[j] == tg[j])
+                    ag.state[i] = kIdle;
+                else
+                    f[j] = 1;
             }
             c += f[j];
         }
@@ -263,48 +296,19 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_fused(DevAgents ag, Dev
 #pragma unroll
         for (int j = 0; j < kAdmitItems; ++j) {
             if (!f[j]) continue;
-            if (r < nfree) {
-                const int i = i0 + j;
-                const int s = fr.free_list[r];
-                sl.agent[s] = i;
-                sl.gen[s] = ag.gen[i];
-                sl.S[s] = ag.curr[i];
-                sl.Tg[s] = ag.targ[i];
-                sl.stamp[s] = sl.stamp[s] + 1u;
-                sl.status[s] = 3;
-                sl.rounds[s] = 0;
-                sl.expansions[s] = 0;
-                sl.discoveries[s] = 0;
-                ag.inflight[i] = s;
-                ag.path_len[i] = 0;  // old corridor released (PathFinder.h:104-108)
-            }
+            if (r < nfree) fr.adm_agent[r] = i0 + j;
             ++r;
         }
         nreq += tot;
     }
     if (mv) atomicAdd(&nmove_sh, mv);
     __syncthreads();
-    int nact = 0;
-    for (int base = 0; base < P; base += kAdmitChunk) {
-        const int s0 = base + tid * kAdmitItems;
-        int f[kAdmitItems];
-        int c = 0;
-#pragma unroll
-        for (int j = 0; j < kAdmitItems; ++j) {
-            f[j] = (s0 + j < P && sl.agent[s0 + j] >= 0) ? 1 : 0;
-            c += f[j];
-        }
-        int tot;
-        int r = nact + admit_scan(c, sh, &tot);
-#pragma unroll
-        for (int j = 0; j < kAdmitItems; ++j)
-            if (f[j]) fr.active[r++] = s0 + j;
-        nact += tot;
-    }
+    // 4
+    const int admitted = nreq < nfree ? nreq : nfree;
+    for (int k = tid; k < admitted; k += kAdmitThreads) fr.active[nocc + k] = fr.free_list[k];
     if (tid == 0) {
-        const int admitted = nreq < nfree ? nreq : nfree;
         fr.counters[0] = nreq;
-        fr.counters[1] = nact;
+        fr.counters[1] = nocc + admitted;
         fr.counters[2] = 0;
         fr.counters[3] = 0;
         fr.counters[4] = nfree;
@@ -312,6 +316,27 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_fused(DevAgents ag, Dev
         fr.counters[6] = admitted;
         fr.stats[6] += (unsigned long long)admitted;
     }
+}
+
+// k-th admission: agent adm_agent[k] into slot free_list[k] (PathFinder.h:104-115)
+__global__ void k_admit_slots(DevAgents ag, DevSlots sl, DevFrame fr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= fr.counters[6]) return;
+    const int s = fr.free_list[k];
+    const int i = fr.adm_agent[k];
+    const int g = ag.gen[i];
+    sl.agent[s] = i;
+    sl.gen[s] = g;
+    sl.S[s] = ag.curr[i];
+    sl.Tg[s] = ag.targ[i];
+    sl.stamp[s] = sl.stamp[s] + 1u;
+    sl.status[s] = 3;
+    sl.rounds[s] = 0;
+    sl.expansions[s] = 0;
+    sl.discoveries[s] = 0;
+    ag.inflight[i] = s;
+    ag.inflight_gen[i] = g;
+    ag.path_len[i] = 0;  // old corridor released (PathFinder.h:104-108)
 }
 
 // GMAPF.cu:247-250 for target changes made after start, read from pinned
@@ -509,7 +534,8 @@ hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& 
                         const DevFrame& fr, hipStream_t st) {
     (void)map;
     hipLaunchKernelGGL(k_apply_updates, dim3(nblk(ag.n > 0 ? ag.n : 1, 256)), dim3(256), 0, st, ag, fr);
-    hipLaunchKernelGGL(k_admit_fused, dim3(1), dim3(kAdmitThreads), 0, st, ag, sl, fr);
+    hipLaunchKernelGGL(k_admit_lists, dim3(1), dim3(kAdmitThreads), 0, st, ag, sl, fr);
+    hipLaunchKernelGGL(k_admit_slots, dim3(nblk(sl.P, 256)), dim3(256), 0, st, ag, sl, fr);
     return hipGetLastError();
 }
 

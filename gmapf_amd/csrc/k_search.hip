@@ -190,6 +190,10 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
     int status = 0;
     uint2* vis = slots.vis;
     int* par = slots.par;
+    // next round's pop, fetched while this round's frontier is pushed
+    int pf_c = -1;
+    int4 pf_nb = make_int4(-1, -1, -1, -1);
+    uint2 pf_vc = make_uint2(0u, 0u);
 
     while (true) {
         // ---- acquire work for segments that have none
@@ -218,6 +222,7 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 disc_here = 0;
                 vis = slots.vis + (size_t)slot * V;
                 par = slots.par + (size_t)slot * V;
+                pf_c = -1;
                 if (status == 3) {
                     // PathFinder.h:168-189
                     const float F0 = dist2(cS, cT) * sp.K;
@@ -251,8 +256,12 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
         int nn[3] = {-1, -1, -1};
         if (popped) {
             c = key_node(heap_pop(q, L, cnt));
-            const int4 nb = map.nbr[c];
-            const uint2 vc = vis[c];
+            int4 nb = pf_nb;
+            uint2 vc = pf_vc;
+            if (c != pf_c) {
+                nb = map.nbr[c];
+                vc = vis[c];
+            }
             G = (int)(vc.y >> 1);
             cdir = (int)(vc.y & 1u);
             nn[0] = nb.x;
@@ -364,8 +373,21 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
         for (int s = 0; s < 3; ++s)
             if (nn[s] >= 0 && !prior[s]) hseg[hslot[s]] = kHashEmpty;
 
-        // ---- deal frontier item f to lane f mod T
+        // ---- deal frontier item f to lane f mod T.  The heap top after the
+        // pushes is min(current top, dealt items) (an item below the top is
+        // never dropped), so next round's node data is requested first and
+        // arrives while the pushes run.
         if (has_req) {
+            unsigned long long nt = cnt > 0 ? q[L] : ~0ull;
+            for (int f = l; f < tot; f += T) {
+                const unsigned long long k = stage[segbase + f];
+                nt = k < nt ? k : nt;
+            }
+            pf_c = nt != ~0ull ? key_node(nt) : -1;
+            if (pf_c >= 0) {
+                pf_nb = map.nbr[pf_c];
+                pf_vc = vis[pf_c];
+            }
             for (int f = l; f < tot; f += T) heap_push(q, L, cnt, Q, stage[segbase + f]);
         }
         const unsigned long long nb = __ballot(cnt > 0) & smask;
