@@ -129,11 +129,179 @@ def cpu_baseline(sc, loc, T, Q, seconds=12.0):
                        f"of the maze workload, single-threaded oracle, {el:.1f} s CPU")
 
 
+def serving_main(args):
+    """Serving workload: rect maps 256^2 / 1024^2 / 4096^2 (1,024 / 16,384 /
+    32,768 agents), Poisson re-path arrivals at 1k, 10k and 50k requests/s,
+    admission by a 2 GiB search-memory budget; one JSON line."""
+    from gmapf_amd.serving import run_serving, serving_maps
+
+    loc = Locators()
+    smaps = serving_maps(loc)
+    T, Q, B = args.T or 8, args.Q or 32, args.budget or 256
+    steps = args.steps if args.steps_given else 300
+    runs = []
+    for lam in (1000.0, 10000.0, 50000.0):
+        r = run_serving(smaps, lam, steps, warmup=args.warmup, T=T, Q=Q, round_budget=B, log=log)
+        log("serving", {k: v for k, v in r.items() if k != "stats"})
+        runs.append(r)
+    top = runs[-1]
+    out = {
+        "metric": "serving loop: path solves/s with p50/p99 frame ms against a 16.6 ms frame budget",
+        "value": round(top["solves_per_s"], 1),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(top["mean_ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+int32",
+        "data": "synthetic (seeded rect maps, uniform free-space starts/targets, PCG64 Poisson arrivals)",
+        "config": {
+            "workload": "serving: 3 device-resident rect maps 256^2/1024^2/4096^2 (400 rects per 10^6 unit^2, "
+                        "sides 5-40) with 1,024/16,384/32,768 agents, Poisson re-path arrivals, "
+                        "2 GiB search-memory admission budget; value at 50k requests/s",
+            "navmesh_nodes": top["nodes"],
+            "T": T, "Q": Q, "round_budget": B,
+            "footprint_bytes_per_request": top["footprint_bytes"],
+            "runs": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()
+                      if k not in ("stats", "nodes", "footprint_bytes")} | {
+                         "admitted": r["stats"]["admitted"], "pending_at_end": r["stats"]["pending"],
+                         "peak_live_bytes": r["stats"]["peak_live_bytes"]} for r in runs],
+            "parallelism": "single GPU, maps on separate streams",
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def crowd_bytes(g, cell, n_moving):
+    """Algorithmic HBM bytes of one crowd phase (move, hash, avoidance,
+    progress) from the hash the frame built: key N*(8+4); histogram and scan
+    N*4 + cells*8; scatter N*20; avoidance N*(8 + s*4 + 8) + 8 per candidate
+    read from the s stencil cells; move/progress 24 per moving agent.  The
+    accept test's triangle reads are not counted."""
+    keys, _ = g.hash_state()
+    tri, _, _ = g.map_arrays()
+    xy = tri.reshape(-1, 2)
+    inv = np.float32(1.0) / np.float32(cell)
+    hnx = int((np.float32(xy[:, 0].max()) - np.float32(xy[:, 0].min())) * inv) + 1
+    hny = int((np.float32(xy[:, 1].max()) - np.float32(xy[:, 1].min())) * inv) + 1
+    k = int(np.floor(np.float32(2.5 * 2.0) / np.float32(cell))) + 1
+    cnt = np.bincount(keys[keys >= 0], minlength=hnx * hny).reshape(hny, hnx).astype(np.int64)
+    pad = np.pad(cnt, k)
+    cs = pad.cumsum(0).cumsum(1)
+    cs = np.pad(cs, ((1, 0), (1, 0)))
+    w = 2 * k + 1
+    box = cs[w:, w:] - cs[:-w, w:] - cs[w:, :-w] + cs[:-w, :-w]
+    cand = int(box.reshape(-1)[keys[keys >= 0]].sum())
+    N = len(keys)
+    s = w * w
+    b = N * 12 + N * 4 + hnx * hny * 8 + N * 20 + N * (16 + 4 * s) + 8 * cand + 24 * n_moving
+    return float(b), cand
+
+
+def crowd_main(args):
+    """Crowd workload: 2048^2 with 50 convex obstacles, 65,536 agents in two
+    opposing streams, hash cell 4; one JSON line (agent-steps/s)."""
+    loc = Locators()
+    sc = S.config4(loc)
+    steps = args.steps if args.steps_given else 600
+    T, Q, B = sc.T, sc.Q, args.budget or 256
+    g = make_world(sc, G.MODE_BATCHED, T, Q, B, hash_cell=4.0)
+    fm = []
+    for f in range(args.warmup + steps):
+        g.async_step(sc.dt)
+        g.wait_step_end()
+        if f >= args.warmup:
+            fm.append(g.last_frame_ms()[0])
+        if f % 100 == 0:
+            log(f"crowd frame {f}: {g.last_frame_ms()[0]:.3f} ms")
+    st_final = g.stats()
+    g.close()
+    fm = np.array(fm)
+    # roofline of the crowd phase: profiled frames (per-phase HIP events)
+    gp = make_world(sc, G.MODE_BATCHED, T, Q, B, profile=True, hash_cell=4.0)
+    tot_b = tot_ms = 0.0
+    cands = []
+    for f in range(40):
+        _, st_before = gp.agents()
+        gp.async_step(sc.dt)
+        gp.wait_step_end()
+        if f >= 10:
+            b, cand = crowd_bytes(gp, 4.0, int((st_before == G.MOVE).sum()))
+            tot_b += b
+            tot_ms += gp.last_frame_ms()[4]
+            cands.append(cand)
+    gp.close()
+    gbs = tot_b / (tot_ms / 1e3) / 1e9
+    roof = dict(bound="hbm", achieved=round(gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(gbs / HBM_PEAK_GBS, 5), traffic=None)
+    cpu = None if args.skip_cpu else crowd_cpu_baseline(sc, loc)
+    out = {
+        "metric": "crowd step: agent-steps/s at 65,536 agents (move + hash + avoidance + progress)",
+        "value": round(sc.n * len(fm) / (fm.sum() / 1e3), 1),
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(float(fm.mean()), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+int32",
+        "data": "synthetic (seeded SplitMix64 convex obstacles and band agents)",
+        "config": {
+            "workload": f"{sc.name}: 50 convex obstacles (6-12 vertices, radius 20-80), {sc.n} agents r=2 v=1 in "
+                        "two opposing streams, hash cell 4 (5x5 stencil), dt 1/60",
+            "p50_ms": round(float(np.percentile(fm, 50)), 4), "p99_ms": round(float(np.percentile(fm, 99)), 4),
+            "crowd_phase_ms": round(tot_ms / 30, 4), "crowd_bytes_per_frame": round(tot_b / 30, 1),
+            "neighbour_candidates_per_frame": int(np.mean(cands)),
+            "searches_found": st_final["found"], "T": T, "Q": Q, "round_budget": B,
+            "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def crowd_cpu_baseline(sc, loc, seconds=12.0):
+    """Single-core oracle world (exact hash avoidance) on the same scenario:
+    frame 0 (the searches) untimed, then frames until ~`seconds` of CPU."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    g = loc.keep[-1]
+    tri, nbr, _ = g.map_arrays()
+    om = pyoracle.OracleMap(tri, nbr)
+    w = pyoracle.OracleWorld(om, agents_max=sc.n, radius=sc.radius, speed=sc.speed, parallel=sc.P,
+                             threads=sc.T, queue_cap=sc.Q, heur_k=sc.K, round_budget=0, hash_cell=4.0,
+                             use_grid_locate=1)
+    for x, y in sc.starts:
+        w.add_agent(x, y)
+    for i, (x, y) in enumerate(sc.targets):
+        w.set_target(i, x, y)
+    w.step(sc.dt)
+    t0 = time.process_time()
+    n = 0
+    while time.process_time() - t0 < seconds:
+        w.step(sc.dt)
+        n += 1
+    el = time.process_time() - t0
+    return dict(value=round(sc.n * n / el, 1), unit="agent-steps/s", cores=1, kind="port",
+                sample=f"{n} frames of the {sc.n}-agent crowd after the search frame, single-threaded oracle "
+                       f"with the exact hash accelerator, {el:.1f} s CPU")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=("maze", "serving", "crowd"), default="maze")
     ap.add_argument("--agents", type=int, default=16384)
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--T", type=int, default=0)
@@ -146,6 +314,13 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
                                    "(DESIGN.md section 8), run with --gpus 1"}))
         sys.exit(2)
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = 120
+    if args.workload == "serving":
+        return serving_main(args)
+    if args.workload == "crowd":
+        return crowd_main(args)
     t_all = time.time()
     loc = Locators()
     sc = S.config3(loc, n_agents=args.agents, cells=args.cells)

@@ -133,6 +133,21 @@ _SIGS = {
     "gmapf_last_frame_ms": [C.c_void_p, C.c_void_p],
     "gmapf_get_hash": [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)],
     "gmapf_get_launch_info": [C.c_void_p, C.c_void_p],
+    "gmapf_get_config": [C.c_void_p, C.POINTER(_Config)],
+    "gmapf_agent_states": [C.c_void_p, C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)],
+    # serving loop (include/gmapf_serve.h)
+    "gmapf_server_create": [C.POINTER(C.c_void_p), C.c_void_p, C.c_int, C.c_void_p],
+    "gmapf_server_destroy": [C.c_void_p],
+    "gmapf_server_submit": [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float, C.POINTER(C.c_longlong)],
+    "gmapf_server_submit_many": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p],
+    "gmapf_server_async_step": [C.c_void_p, C.c_float],
+    "gmapf_server_wait_step_end": [C.c_void_p],
+    "gmapf_server_poll": [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)],
+    "gmapf_server_plan": [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)],
+    "gmapf_server_get_stats": [C.c_void_p, C.c_void_p],
+    "gmapf_server_last_frame_ms": [C.c_void_p, C.c_void_p],
+    "gmapf_server_footprint": [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)],
 }
 
 
@@ -151,6 +166,8 @@ def load_library(path: str | None = None):
         fn.restype = C.c_int
     lib.gmapf_last_error.argtypes = [C.c_void_p]
     lib.gmapf_last_error.restype = C.c_char_p
+    lib.gmapf_server_last_error.argtypes = [C.c_void_p]
+    lib.gmapf_server_last_error.restype = C.c_char_p
     if path is None:
         _LIB = lib
     return lib
@@ -274,6 +291,24 @@ class GMAPF:
         n = C.c_int()
         self._check(self._lib.gmapf_agent_count(self._h, C.byref(n)))
         return n.value
+
+    def get_config(self) -> Config:
+        c = _Config()
+        self._check(self._lib.gmapf_get_config(self._h, C.byref(c)))
+        kw = {}
+        for f in fields(Config):
+            v = getattr(c, f.name)
+            kw[f.name] = bool(v) if f.type in (bool, "bool") else v
+        return Config(**kw)
+
+    def agent_states(self) -> np.ndarray:
+        """Copy of the host state mirror (refreshed by every wait_step_end)."""
+        p = C.POINTER(C.c_int)()
+        n = C.c_int()
+        self._check(self._lib.gmapf_agent_states(self._h, C.byref(p), C.byref(n)))
+        if n.value == 0:
+            return np.zeros(0, dtype=np.int32)
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy()
 
     def agents(self, with_nodes=False):
         n = self.agent_count()

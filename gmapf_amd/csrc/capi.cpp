@@ -832,6 +832,22 @@ int gmapf_agent_count(gmapf_handle h, int* n) {
     });
 }
 
+int gmapf_get_config(gmapf_handle h, gmapf_config* out) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!out) throw ApiError(GMAPF_E_ARG, "null output");
+        *out = h->cfg;
+    });
+}
+
+int gmapf_agent_states(gmapf_handle h, const int** st, int* n) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (st) *st = h->state.data();
+        if (n) *n = h->n_agents;
+    });
+}
+
 int gmapf_get_agents_soa(gmapf_handle h, float* xy, int* st, int* node) {
     return guarded(h, [&] {
         h->require_init();

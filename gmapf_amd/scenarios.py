@@ -219,6 +219,11 @@ def rect_obstacles(seed, W, H, n, smin, smax, gap=2.0):
     1-unit snap tolerance)."""
     r = stream(seed, 21)
     rects = []
+    # accepted rects bucketed by the cell of their lower-left corner: a rect
+    # that can touch a candidate has its corner within smax + gap of the
+    # candidate's corner, so the 3x3 cells around it hold every such rect
+    cs = smax + gap
+    buckets = {}
     tries = 0
     while len(rects) < n and tries < 200000:
         tries += 1
@@ -227,9 +232,12 @@ def rect_obstacles(seed, W, H, n, smin, smax, gap=2.0):
         x = gap + (W - w - 2 * gap) * r.uniform()
         y = gap + (H - h - 2 * gap) * r.uniform()
         x, y, w, h = (round(v * 16) / 16 for v in (x, y, w, h))  # 1/16-unit grid
-        if any(x < b[2] + gap and b[0] < x + w + gap and y < b[3] + gap and b[1] < y + h + gap for b in rects):
+        cx, cy = int(x // cs), int(y // cs)
+        near = (b for i in (cx - 1, cx, cx + 1) for j in (cy - 1, cy, cy + 1) for b in buckets.get((i, j), ()))
+        if any(x < b[2] + gap and b[0] < x + w + gap and y < b[3] + gap and b[1] < y + h + gap for b in near):
             continue
         rects.append((x, y, x + w, y + h))
+        buckets.setdefault((cx, cy), []).append(rects[-1])
     return [np.array([(a, b), (a, d), (c, d), (c, b)], np.float32) for a, b, c, d in rects]
 
 

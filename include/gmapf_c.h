@@ -98,6 +98,11 @@ int gmapf_profiler_dump(gmapf_handle h);                               /* GMAPF.
 
 /* ---- extensions (no reference counterpart): bulk, introspection, bench */
 int gmapf_agent_count(gmapf_handle h, int* n);
+/* the configuration given to gmapf_init */
+int gmapf_get_config(gmapf_handle h, gmapf_config* out);
+/* Host mirror of all states (GMAPF_STATE_*), valid until destroy, refreshed
+ * by every WaitStepEnd. */
+int gmapf_agent_states(gmapf_handle h, const int** state, int* n);
 int gmapf_get_agents_soa(gmapf_handle h, float* xy, int* state, int* curr_node);
 int gmapf_set_agent_targ_pos_many(gmapf_handle h, int n, const int* ids, const float* xy,
                                   int* status /* per item 0 or GMAPF_E_POS, may be NULL */);

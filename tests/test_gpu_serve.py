@@ -1,0 +1,158 @@
+"""Serving loop on the GPU: every request's result in the mix (several maps,
+continuous Poisson arrivals, searches spanning frames and overlapping with
+new admissions under a memory budget) equals its result alone on the device
+and the oracle's search, and the server's counters stay consistent."""
+import numpy as np
+import pytest
+
+import gmapf_amd as G
+from gmapf_amd.runner import Locators
+from gmapf_amd.serving import Server, build_map, serving_maps
+
+pytestmark = pytest.mark.gpu
+
+T, Q = 8, 32
+
+
+def _maps(locs):
+    # two small rect maps at the serving workload's obstacle density
+    return serving_maps(locs, seed=11, sizes=(160.0, 320.0), agents=(200, 400), slots=(24, 48),
+                        target_pool=4096)
+
+
+def _drive(srv, smaps, engines, frames, lam, seed, corridors):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    done_all = []
+    for f in range(frames):
+        k = int(rng.poisson(lam)) if lam else 0
+        if k:
+            mi = rng.integers(0, len(smaps), size=k).astype(np.int32)
+            ag = np.array([rng.integers(0, smaps[m].n_agents) for m in mi], dtype=np.int32)
+            xy = np.stack([smaps[m].targets[rng.integers(0, len(smaps[m].targets))] for m in mi])
+            _, st = srv.submit_many(mi, ag, xy)
+            assert (st == 0).all()
+            for m, a, p in zip(mi, ag, xy):
+                corridors.setdefault("targets", {}).setdefault((int(m), int(a)), []).append(p)
+        if f % 50 == 0:
+            print(f"serve frame {f} submitted {k}", flush=True)
+        srv.step(1.0 / 60)
+        done = srv.poll()
+        for d in done:
+            if d["found"]:
+                c = engines[d["map"]].corridor(int(d["agent"]))
+                corridors[int(d["ticket"])] = None if c is None else c.copy()
+        done_all.append(done)
+        s = srv.stats()
+        assert s["submitted"] == s["admitted"] + s["pending"] + s["rejected"]
+        assert s["completed"] == s["found"] + s["no_path"]
+        assert s["admitted"] == s["completed"] + s["in_flight"]
+        if not lam and s["pending"] == 0 and s["in_flight"] == 0:
+            break
+    return np.concatenate(done_all)
+
+
+def test_results_alone_equal_results_in_the_mix(oracle):
+    locs = Locators()
+    smaps = _maps(locs)
+    engines = [build_map(sm, T, Q, round_budget=3) for sm in smaps]
+    fb = [e.launch_info()["V"] * 12 + T * Q * 8 + T * 4 for e in engines]
+    budget = 12 * fb[1] + 12 * fb[0]  # fewer than the slots: the budget binds
+    srv = Server(engines, queue_memory_budget=budget, record_start_nodes=True)
+    assert [srv.footprint(m) for m in range(2)] == fb
+    log = {}
+    # requests keep arriving while earlier ones are searching (round budget 3)
+    done = _drive(srv, smaps, engines, 40, 12.0, 5, log)
+    # drain: no more arrivals
+    more = _drive(srv, smaps, engines, 5000, 0, 6, log)
+    done = np.concatenate([done, more])
+    st = srv.stats()
+    assert st["pending"] == 0 and st["in_flight"] == 0, st
+    assert st["completed"] == st["submitted"] == len(done)
+    assert st["peak_live_bytes"] <= budget
+    assert (done["admit_frame"] > done["submit_frame"]).any(), "the budget never held a request back"
+    assert (done["done_frame"] > done["admit_frame"]).any(), "no search spanned frames"
+    assert done["found"].sum() > 0.5 * len(done)
+
+    # alone: one request at a time on a fresh engine of the same map, agent
+    # placed on the centroid of the request's start node
+    rng = np.random.Generator(np.random.PCG64(9))
+    pick = rng.choice(len(done), size=min(40, len(done)), replace=False)
+    by_map = {}
+    for i in pick:
+        by_map.setdefault(int(done[i]["map"]), []).append(done[i])
+    # ticket -> target position, in submission order per (map, agent)
+    tickets_by_key = {}
+    for d in sorted(done, key=lambda d: d["ticket"]):
+        tickets_by_key.setdefault((int(d["map"]), int(d["agent"])), []).append(int(d["ticket"]))
+    target_of = {}
+    for key, ts in tickets_by_key.items():
+        for t, p in zip(ts, log["targets"][key]):
+            target_of[t] = p
+    checked = 0
+    for m, reqs in by_map.items():
+        sm = smaps[m]
+        tri, nbr, cen = engines[m].map_arrays()
+        om = oracle.OracleMap(tri, nbr)
+        starts = np.array([cen[int(d["start_node"])] for d in reqs], dtype=np.float32)
+        alone = G.GMAPF()
+        alone.init(G.Config(agents_max_count=len(reqs), agent_radius=2.0, agent_speed=1.0,
+                            path_finder_parallel_agents=1, path_finder_threads_per_agents=T,
+                            path_finder_queue_capacity=Q, step_mode=G.MODE_PLAIN))
+        alone.fill_map_manually(tri, nbr)
+        for x, y in starts:
+            alone.add_agent(x, y)
+        _, _, nodes = alone.agents(with_nodes=True)
+        assert list(nodes) == [int(d["start_node"]) for d in reqs]
+        alone.manual_start()
+        for k, d in enumerate(reqs):
+            tx, ty = target_of[int(d["ticket"])]
+            print(f"alone map {m} ticket {int(d['ticket'])}", flush=True)
+            alone.set_agent_targ_pos(k, tx, ty)
+            for _ in range(10):
+                alone.step(1.0 / 60)
+                if alone.get_agent_state(k) != G.SEARCH:
+                    break
+            assert alone.get_agent_state(k) != G.SEARCH
+            mix = log.get(int(d["ticket"]))
+            got = alone.corridor(k) if alone.get_agent_state(k) == G.MOVE else None
+            assert (got is not None) == bool(d["found"])
+            tg = int(om.locate(np.array([[tx, ty]], np.float32))[0])
+            S_node = int(d["start_node"])
+            if S_node != tg:
+                of, oc, _ = om.search(S_node, tg, T, Q)
+                assert of == bool(d["found"])
+                if of:
+                    np.testing.assert_array_equal(oc, got)
+            if d["found"]:
+                np.testing.assert_array_equal(mix, got)
+            # park the agent: Idle agents keep no request
+            checked += 1
+        alone.close()
+    assert checked == len(pick)
+    srv.close()
+    for e in engines:
+        e.close()
+
+
+def test_three_maps_overlap_and_budget_bounds_concurrency():
+    locs = Locators()
+    smaps = serving_maps(locs, seed=2, sizes=(128.0, 256.0, 512.0), agents=(128, 512, 1024),
+                         slots=(64, 128, 128), target_pool=4096)
+    engines = [build_map(sm, 8, 16, round_budget=32) for sm in smaps]
+    probe = Server(engines)
+    fp = [probe.footprint(m) for m in range(3)]
+    probe.close()
+    budget = 100 * fp[2]
+    srv = Server(engines, queue_memory_budget=budget)
+    log = {}
+    _drive(srv, smaps, engines, 60, 20.0, 3, log)
+    _drive(srv, smaps, engines, 5000, 0, 4, log)
+    st = srv.stats()
+    assert st["pending"] == 0 and st["in_flight"] == 0
+    assert st["completed"] == st["submitted"] > 900
+    assert 0 < st["peak_live_bytes"] <= budget
+    ms = srv.last_frame_ms()
+    assert ms[0] > 0 and ms[3] >= 0
+    srv.close()
+    for e in engines:
+        e.close()
