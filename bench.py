@@ -140,7 +140,7 @@ def serving_main(args):
     T, Q, B = args.T or 8, args.Q or 32, args.budget or 256
     steps = args.steps if args.steps_given else 300
     runs = []
-    for lam in (1000.0, 10000.0, 50000.0):
+    for lam in ([args.lam] if args.lam else (1000.0, 10000.0, 50000.0)):
         r = run_serving(smaps, lam, steps, warmup=args.warmup, T=T, Q=Q, round_budget=B, log=log)
         log("serving", {k: v for k, v in r.items() if k != "stats"})
         runs.append(r)
@@ -309,6 +309,7 @@ def main():
     ap.add_argument("--budget", type=int, default=0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--lam", type=float, default=0.0, help="serving: one arrival rate (requests/s) only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "

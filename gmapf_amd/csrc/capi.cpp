@@ -292,7 +292,6 @@ struct gmapf_s {
         dag.path = dalloc<int>((size_t)Nn * V, "agent corridors");
         dag.gen = dalloc<int>(Nn, "agent gen");
         dag.inflight = dalloc<int>(Nn, "agent inflight");
-        dag.inflight_gen = dalloc<int>(Nn, "agent inflight gen");
         dag.in_m = dalloc<int>(Nn, "agent moving flag");
         if (N > 0) {
             std::vector<float2> tp(N);
@@ -308,7 +307,6 @@ struct gmapf_s {
         HIPCHK(launch_fill(dag.step_node, -1, Nn, s1));
         HIPCHK(launch_fill(dag.path_len, 0, Nn, s1));
         HIPCHK(launch_fill(dag.inflight, -1, Nn, s1));
-        HIPCHK(launch_fill(dag.inflight_gen, 0, Nn, s1));
         HIPCHK(launch_fill(dag.in_m, 0, Nn, s1));
         HIPCHK(hipMemsetAsync(dag.step_pos, 0, Nn * sizeof(float2), s1));
 
@@ -346,6 +344,7 @@ struct gmapf_s {
         dfr.sflags = dalloc<int>(P, "slot flags");
         dfr.sranks = dalloc<int>(P, "slot ranks");
         dfr.blk = dalloc<int>(Nn / scan_block + 2, "scan partials");
+        dfr.mblk = dalloc<int>(Nn / scan_block + 2, "moving partials");
         dfr.sblk = dalloc<int>(P / scan_block + 2, "slot scan partials");
         dfr.free_list = dalloc<int>(P, "free slots");
         dfr.adm_agent = dalloc<int>(P, "admitted agents");

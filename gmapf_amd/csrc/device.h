@@ -62,7 +62,6 @@ struct DevAgents {
     long long path_cap;
     int* gen;
     int* inflight;
-    int* inflight_gen;  // gen of the agent when its in-flight search was admitted
     int* in_m;
 };
 
@@ -92,14 +91,16 @@ struct DevFrame {
     int* ranks;      // N
     int* sflags;     // P   slot flags
     int* sranks;     // P
-    int* blk;        // block partials (agents)
-    int* sblk;       // block partials (slots)
+    int* blk;        // per-block request counts, then their exclusive prefixes
+    int* mblk;       // per-block moving-agent counts
+    int* sblk;       // per-block free-slot counts, then their exclusive prefixes
     int* free_list;  // P
     int* adm_agent;  // P: agent of the k-th admission this frame
     int* active;     // P: active slot list for the search kernel
     int* ended;      // P: slots that ended this frame
     int* counters;   // [0] n_requests [1] n_active [2] work ticket [3] n_ended
                      // [4] n_free [5] moving count [6] n_admitted
+                     // [7] blocks done in the admission count pass
     unsigned long long* stats;  // [0] found [1] fail [2] expansions [3] discoveries
                                 // [4] corridor nodes [5] rounds [6] admitted
     // spatial hash

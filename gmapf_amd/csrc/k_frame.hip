@@ -163,11 +163,28 @@ __device__ int locate(const DevMap& m, float x, float y) {
 }
 
 // ------------------------------------------------------------- admission
+// F1 (PathFinder.h:85-116) and F2 (AgentsMover.h:39-46) as two grid-wide
+// passes, one agent or slot per thread (blocks [0, nbA) agents, the rest
+// slots):
+//   count:   a Search agent with no search in flight is a request, or turns
+//            Idle when its node equals its target node (PathFinder.h:91-95);
+//            Move agents form the moving set; a slot with no agent is free.
+//            Block totals go to blk (requests), mblk (moving) and sblk (free
+//            slots); the last block to finish turns blk and sblk into
+//            exclusive prefixes and writes the frame's counts.
+//   scatter: the r-th request in ascending agent index is admitted when
+//            r < number of free slots (the rest stay Search,
+//            PathFinder.h:97-102); free slots are listed ascending, occupied
+//            ones start the active list.
+// k_admit_slots initialises the admitted searches, the r-th request in the
+// r-th free slot (PathFinder.h:104-115), and appends their slots to the
+// active list.  A search whose agent got a new target was dropped before,
+// by k_apply_updates.
 constexpr int kAdmitThreads = 1024;
 constexpr int kAdmitWaves = kAdmitThreads / 64;
 
-// exclusive scan of one value per thread over the 1024-thread block; returns
-// the thread's prefix and the block total.  Starts with a barrier, so
+// exclusive scan of one value per thread over the block; returns the
+// thread's prefix and the block total.  Starts with a barrier, so
 // consecutive calls may reuse sh.
 __device__ __forceinline__ int admit_scan(int v, int* sh, int* total) {
     const int ln = threadIdx.x % 64, w = threadIdx.x / 64;
@@ -189,144 +206,131 @@ __device__ __forceinline__ int admit_scan(int v, int* sh, int* total) {
     return wpre + x - v;
 }
 
-// F1 (PathFinder.h:85-116) and F2 (AgentsMover.h:39-46), list building in one
-// workgroup with coalesced accesses only:
-//   1. agents whose target changed while their search was in flight drop it;
-//   2. occupied slots (ascending) start the active list, free slots are
-//      listed in ascending slot index;
-//   3. requests in ascending agent index = Search agents not in flight; a
-//      start node equal to the target node makes the agent Idle
-//      (PathFinder.h:91-95); the r-th request takes the r-th free slot, the
-//      rest stay Search (PathFinder.h:97-102); the moving set = agents in Move;
-//   4. the admitted slots complete the active list.
-// The scattered per-slot initialisation of the admitted searches is done by
-// k_admit_slots across the whole GPU.  Each thread owns kAdmitItems
-// consecutive indices of a chunk; all loads of a chunk are issued first.
-constexpr int kAdmitItems = 8;
-constexpr int kAdmitChunk = kAdmitThreads * kAdmitItems;
-
-__global__ __launch_bounds__(kAdmitThreads) void k_admit_lists(DevAgents ag, DevSlots sl, DevFrame fr) {
-    __shared__ int sh[kAdmitWaves];
-    __shared__ int nmove_sh;
-    const int tid = threadIdx.x;
-    const int P = sl.P, N = ag.n;
-    if (tid == 0) {
-        *fr.stepv = *fr.h_step;
-        nmove_sh = 0;
-    }
-    // 1
-    for (int base = 0; base < N; base += kAdmitChunk) {
-        const int i0 = base + tid * kAdmitItems;
-        int inf[kAdmitItems], g[kAdmitItems], ig[kAdmitItems];
-#pragma unroll
-        for (int j = 0; j < kAdmitItems; ++j) {
-            const bool ok = i0 + j < N;
-            inf[j] = ok ? ag.inflight[i0 + j] : -1;
-            g[j] = ok ? ag.gen[i0 + j] : 0;
-            ig[j] = ok ? ag.inflight_gen[i0 + j] : 0;
-        }
-#pragma unroll
-        for (int j = 0; j < kAdmitItems; ++j) {
-            if (inf[j] >= 0 && g[j] != ig[j]) {
-                sl.agent[inf[j]] = -1;
-                ag.inflight[i0 + j] = -1;
-            }
-        }
-    }
+// exclusive rank of a flag within the block and the block's flag count;
+// wsum must not be in use by another call still being read
+__device__ __forceinline__ int admit_rank(bool f, int* wsum, int* total) {
+    const int ln = threadIdx.x % 64, w = threadIdx.x / 64;
+    const unsigned long long bal = __ballot(f);
+    if (ln == 0) wsum[w] = __popcll(bal);
     __syncthreads();
-    // 2
-    int nfree = 0, nocc = 0;
-    for (int base = 0; base < P; base += kAdmitChunk) {
-        const int s0 = base + tid * kAdmitItems;
-        int f[kAdmitItems];
-        int c = 0;
-#pragma unroll
-        for (int j = 0; j < kAdmitItems; ++j) {
-            f[j] = (s0 + j < P) ? (sl.agent[s0 + j] < 0 ? 1 : 2) : 0;
-            c += (f[j] == 1) ? 1 : 0;
-        }
-        int tot;
-        int r = nfree + admit_scan(c, sh, &tot);
-        // occupied slots before this thread's items = valid items before it - free ones
-        int o = nocc + tid * kAdmitItems - (r - nfree);
-#pragma unroll
-        for (int j = 0; j < kAdmitItems; ++j) {
-            if (f[j] == 1)
-                fr.free_list[r++] = s0 + j;
-            else if (f[j] == 2)
-                fr.active[o++] = s0 + j;
-        }
-        const int chunk_items = (P - base) < kAdmitChunk ? (P - base) : kAdmitChunk;
-        nfree += tot;
-        nocc += chunk_items - tot;
+    int pre = 0, tot = 0;
+    for (int k = 0; k < kAdmitWaves; ++k) {
+        const int t = wsum[k];
+        pre += (k < w) ? t : 0;
+        tot += t;
     }
-    __syncthreads();  // free list visible to all threads
-    // 3
-    int nreq = 0, mv = 0;
-    for (int base = 0; base < N; base += kAdmitChunk) {
-        const int i0 = base + tid * kAdmitItems;
-        int st[kAdmitItems], inf[kAdmitItems], cu[kAdmitItems], tg[kAdmitItems];
-#pragma unroll
-        for (int j = 0; j < kAdmitItems; ++j) {
-            const bool ok = i0 + j < N;
-            st[j] = ok ? ag.state[i0 + j] : kIdle;
-            inf[j] = ok ? ag.inflight[i0 + j] : 0;
-            cu[j] = ok ? ag.curr[i0 + j] : 0;
-            tg[j] = ok ? ag.targ[i0 + j] : 0;
-        }
-        int f[kAdmitItems];
-        int c = 0;
-#pragma unroll
-        for (int j = 0; j < kAdmitItems; ++j) {
-            const int i = i0 + j;
-            f[j] = 0;
-            if (i >= N) continue;
-            ag.in_m[i] = (st[j] == kMove) ? 1 : 0;
-            mv += (st[j] == kMove) ? 1 : 0;
-            if (st[j] == kSearch && inf[j] < 0) {
-                if (cu[j] == tg[j])
+    *total = tot;
+    return pre + __popcll(bal & ((1ull << ln) - 1ull));
+}
+
+__device__ __forceinline__ int load_agent_scope(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// exclusive prefix over vals[0..n) in place by one block; returns the total
+__device__ int admit_prefix(int* vals, int n, int* sh) {
+    int carry = 0;
+    for (int base = 0; base < n; base += kAdmitThreads) {
+        const int k = base + threadIdx.x;
+        const int v = k < n ? load_agent_scope(vals + k) : 0;
+        int tot;
+        const int pre = admit_scan(v, sh, &tot);
+        if (k < n) vals[k] = carry + pre;
+        carry += tot;
+    }
+    return carry;
+}
+
+__global__ __launch_bounds__(kAdmitThreads) void k_admit_count(DevAgents ag, DevSlots sl, DevFrame fr, int nbA) {
+    __shared__ int wsum_f[kAdmitWaves], wsum_m[kAdmitWaves], sh[kAdmitWaves];
+    __shared__ int is_last;
+    const int b = blockIdx.x;
+    bool f = false, mv = false;
+    if (b < nbA) {
+        const int i = b * kAdmitThreads + threadIdx.x;
+        if (i < ag.n) {
+            const int st = ag.state[i];
+            mv = st == kMove;
+            ag.in_m[i] = mv ? 1 : 0;
+            if (st == kSearch && ag.inflight[i] < 0) {
+                if (ag.curr[i] == ag.targ[i])
                     ag.state[i] = kIdle;
                 else
-                    f[j] = 1;
+                    f = true;
             }
-            c += f[j];
         }
-        int tot;
-        int r = nreq + admit_scan(c, sh, &tot);
-#pragma unroll
-        for (int j = 0; j < kAdmitItems; ++j) {
-            if (!f[j]) continue;
-            if (r < nfree) fr.adm_agent[r] = i0 + j;
-            ++r;
-        }
-        nreq += tot;
+    } else {
+        const int s = (b - nbA) * kAdmitThreads + threadIdx.x;
+        f = s < sl.P && sl.agent[s] < 0;
     }
-    if (mv) atomicAdd(&nmove_sh, mv);
+    int nf, nm;
+    admit_rank(f, wsum_f, &nf);
+    admit_rank(mv, wsum_m, &nm);
+    if (threadIdx.x == 0) {
+        if (b < nbA) {
+            fr.blk[b] = nf;
+            fr.mblk[b] = nm;
+        } else {
+            fr.sblk[b - nbA] = nf;
+        }
+        __threadfence();
+        is_last = atomicAdd(&fr.counters[7], 1) == (int)gridDim.x - 1;
+    }
     __syncthreads();
-    // 4
-    const int admitted = nreq < nfree ? nreq : nfree;
-    for (int k = tid; k < admitted; k += kAdmitThreads) fr.active[nocc + k] = fr.free_list[k];
-    if (tid == 0) {
+    if (!is_last) return;
+    __threadfence();
+    const int nbS = (int)gridDim.x - nbA;
+    const int nreq = admit_prefix(fr.blk, nbA, sh);
+    const int nfree = admit_prefix(fr.sblk, nbS, sh);
+    int mv_part = 0;
+    for (int k = threadIdx.x; k < nbA; k += kAdmitThreads) mv_part += load_agent_scope(fr.mblk + k);
+    int moving;
+    admit_scan(mv_part, sh, &moving);
+    if (threadIdx.x == 0) {
+        const int admitted = nreq < nfree ? nreq : nfree;
         fr.counters[0] = nreq;
-        fr.counters[1] = nocc + admitted;
+        fr.counters[1] = (sl.P - nfree) + admitted;
         fr.counters[2] = 0;
         fr.counters[3] = 0;
         fr.counters[4] = nfree;
-        fr.counters[5] = nmove_sh;
+        fr.counters[5] = moving;
         fr.counters[6] = admitted;
+        fr.counters[7] = 0;  // every block has counted itself: ready for the next frame
         fr.stats[6] += (unsigned long long)admitted;
+    }
+}
+
+__global__ __launch_bounds__(kAdmitThreads) void k_admit_scatter(DevAgents ag, DevSlots sl, DevFrame fr, int nbA) {
+    __shared__ int wsum[kAdmitWaves];
+    const int b = blockIdx.x;
+    int tot;
+    if (b < nbA) {
+        const int i = b * kAdmitThreads + threadIdx.x;
+        const bool f = i < ag.n && ag.state[i] == kSearch && ag.inflight[i] < 0;
+        const int r = fr.blk[b] + admit_rank(f, wsum, &tot);
+        if (f && r < fr.counters[4]) fr.adm_agent[r] = i;
+    } else {
+        const int s = (b - nbA) * kAdmitThreads + threadIdx.x;
+        const bool f = s < sl.P && sl.agent[s] < 0;
+        const int r = fr.sblk[b - nbA] + admit_rank(f, wsum, &tot);
+        if (s < sl.P) {
+            if (f)
+                fr.free_list[r] = s;
+            else
+                fr.active[s - r] = s;  // s - r = occupied slots before s
+        }
     }
 }
 
 // k-th admission: agent adm_agent[k] into slot free_list[k] (PathFinder.h:104-115)
 __global__ void k_admit_slots(DevAgents ag, DevSlots sl, DevFrame fr) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= fr.counters[6]) return;
+    const int admitted = fr.counters[6];
+    if (k >= admitted) return;
     const int s = fr.free_list[k];
     const int i = fr.adm_agent[k];
-    const int g = ag.gen[i];
     sl.agent[s] = i;
-    sl.gen[s] = g;
+    sl.gen[s] = ag.gen[i];
     sl.S[s] = ag.curr[i];
     sl.Tg[s] = ag.targ[i];
     sl.stamp[s] = sl.stamp[s] + 1u;
@@ -335,15 +339,20 @@ __global__ void k_admit_slots(DevAgents ag, DevSlots sl, DevFrame fr) {
     sl.expansions[s] = 0;
     sl.discoveries[s] = 0;
     ag.inflight[i] = s;
-    ag.inflight_gen[i] = g;
     ag.path_len[i] = 0;  // old corridor released (PathFinder.h:104-108)
+    fr.active[fr.counters[1] - admitted + k] = s;
 }
 
 // GMAPF.cu:247-250 for target changes made after start, read from pinned
 // host memory; the host passes at most one update per agent (the last one).
-__global__ void k_apply_updates(DevAgents ag, DevFrame fr) {
+// An agent whose search is in flight drops it: its slot is free for this
+// frame's admission, and the agent requests again with the new target.
+__global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
     __shared__ int n_sh;
-    if (threadIdx.x == 0) n_sh = *fr.h_upd_n;
+    if (threadIdx.x == 0) {
+        n_sh = *fr.h_upd_n;
+        if (blockIdx.x == 0) *fr.stepv = *fr.h_step;
+    }
     __syncthreads();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_sh) return;
@@ -352,6 +361,11 @@ __global__ void k_apply_updates(DevAgents ag, DevFrame fr) {
     ag.targ[i] = fr.h_upd_targ[k];
     ag.targ_pos[i] = fr.h_upd_xy[k];
     ag.gen[i] = ag.gen[i] + 1;
+    const int s = ag.inflight[i];
+    if (s >= 0) {
+        sl.agent[s] = -1;
+        ag.inflight[i] = -1;
+    }
 }
 
 // positions, states and counters of the finished frame into pinned host memory
@@ -533,8 +547,10 @@ hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, long lon
 hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& sl,
                         const DevFrame& fr, hipStream_t st) {
     (void)map;
-    hipLaunchKernelGGL(k_apply_updates, dim3(nblk(ag.n > 0 ? ag.n : 1, 256)), dim3(256), 0, st, ag, fr);
-    hipLaunchKernelGGL(k_admit_lists, dim3(1), dim3(kAdmitThreads), 0, st, ag, sl, fr);
+    const int nbA = nblk(ag.n, kAdmitThreads), nbS = nblk(sl.P, kAdmitThreads);
+    hipLaunchKernelGGL(k_apply_updates, dim3(nblk(ag.n > 0 ? ag.n : 1, 256)), dim3(256), 0, st, ag, sl, fr);
+    hipLaunchKernelGGL(k_admit_count, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
+    hipLaunchKernelGGL(k_admit_scatter, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
     hipLaunchKernelGGL(k_admit_slots, dim3(nblk(sl.P, 256)), dim3(256), 0, st, ag, sl, fr);
     return hipGetLastError();
 }
