@@ -214,6 +214,28 @@ struct gmapf_s {
             for (int k = 0; k < 3; k++)
                 tri2[3 * (size_t)i + k] = make_float2(nav.tri[6 * (size_t)i + 2 * k], nav.tri[6 * (size_t)i + 2 * k + 1]);
         }
+        // search records: neighbours + their centroids (bit patterns) per node
+        std::vector<int4> rec(3 * (size_t)V);
+        auto fbits = [](float f) {
+            int b;
+            std::memcpy(&b, &f, sizeof(b));
+            return b;
+        };
+        for (int i = 0; i < V; i++) {
+            const int4 nb = nbr4[i];
+            const int n[3] = {nb.x, nb.y, nb.z};
+            float c[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int k = 0; k < 3; k++)
+                if (n[k] >= 0) {
+                    c[2 * k] = cen2[n[k]].x;
+                    c[2 * k + 1] = cen2[n[k]].y;
+                }
+            rec[3 * (size_t)i] = make_int4(n[0], n[1], n[2], -1);
+            rec[3 * (size_t)i + 1] = make_int4(fbits(c[0]), fbits(c[1]), fbits(c[2]), fbits(c[3]));
+            rec[3 * (size_t)i + 2] = make_int4(fbits(c[4]), fbits(c[5]), 0, 0);
+        }
+        int4* d_rec = dalloc<int4>(3 * (size_t)V, "navmesh search records");
+        HIPCHK(hipMemcpy(d_rec, rec.data(), 3 * (size_t)V * sizeof(int4), hipMemcpyHostToDevice));
         int4* d_nbr = dalloc<int4>(V, "navmesh neighbours");
         float2* d_cen = dalloc<float2>(V, "navmesh centroids");
         float2* d_tri = dalloc<float2>(3 * (size_t)V, "navmesh triangles");
@@ -233,6 +255,7 @@ struct gmapf_s {
         dmap.nbr = d_nbr;
         dmap.cen = d_cen;
         dmap.tri = d_tri;
+        dmap.rec = d_rec;
         dmap.gx0 = nav.gx0;
         dmap.gy0 = nav.gy0;
         dmap.inv_cs = nav.inv_cs;
@@ -324,14 +347,13 @@ struct gmapf_s {
         dsl.rounds = dalloc<long long>(P, "slot rounds");
         dsl.expansions = dalloc<long long>(P, "slot expansions");
         dsl.discoveries = dalloc<long long>(P, "slot discoveries");
-        dsl.vis = dalloc<uint2>((size_t)P * V, "visited tables");
-        dsl.par = dalloc<int>((size_t)P * V, "parent tables");
+        dsl.vis = dalloc<uint4>((size_t)P * V, "visited tables");
         dsl.qsave = dalloc<unsigned long long>((size_t)P * T * Q, "parked queues");
         dsl.qcnt = dalloc<int>((size_t)P * T, "parked queue counts");
         HIPCHK(launch_fill(dsl.agent, -1, P, s1));
         HIPCHK(launch_fill(dsl.status, 0, P, s1));
         HIPCHK(hipMemsetAsync(dsl.stamp, 0, P * sizeof(unsigned), s1));
-        HIPCHK(hipMemsetAsync(dsl.vis, 0, (size_t)P * V * sizeof(uint2), s1));
+        HIPCHK(hipMemsetAsync(dsl.vis, 0, (size_t)P * V * sizeof(uint4), s1));
         HIPCHK(hipMemsetAsync(dsl.rounds, 0, P * sizeof(long long), s1));
         HIPCHK(hipMemsetAsync(dsl.expansions, 0, P * sizeof(long long), s1));
         HIPCHK(hipMemsetAsync(dsl.discoveries, 0, P * sizeof(long long), s1));

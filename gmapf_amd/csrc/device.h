@@ -5,6 +5,9 @@
 //   nbr[V]   int4  {n0, n1, n2, unused}: compacted neighbours, -1 tail
 //            (reference Node::NeibsIdx, CuNodesMap.h:14-21, GMAPF.cu:185-190)
 //   cen[V]   float2 centroid ((P1+P2)+P3)/3 (GMAPF.cu:215)
+//   rec[3V]  int4 x 3 per node, the search's view of it in one 48-byte read:
+//            {n0, n1, n2, -1}, {bits of cen[n0], cen[n1]}, {bits of cen[n2], 0, 0}
+//            (a missing neighbour's centroid is 0)
 //   tri[V]   3 x float2 vertices, for point-in-triangle tests
 //   point-location grid: cell_start/cell_tris (ascending triangle ids per
 //   cell, triangles listed in every cell their margin-expanded bbox touches)
@@ -15,10 +18,11 @@
 //   path[id * path_cap ...] corridor, gen int (bumped by SetAgentTargPos),
 //   inflight int (search slot or -1), in_m int (moving set of this frame).
 // Search slots (P of them, PathFinderParallelAgents):
-//   vis[slot * V + node] uint2 {stamp, G << 1 | dir} and par[slot * V + node]
-//   (parent node, read only by the corridor walk); an entry belongs to the
-//   current search of the slot only when its stamp equals slot_stamp[slot],
-//   so slots are never cleared (replaces PathFinder.h:119-136).
+//   vis[slot * V + node] uint4 {stamp, G << 1 | dir, parent, 0}: one aligned
+//   16-byte entry, so a discovery is one store to one cache line; an entry
+//   belongs to the current search of the slot only when its stamp equals
+//   slot_stamp[slot], so slots are never cleared (replaces
+//   PathFinder.h:119-136).
 //   qsave[slot][lane][Q] uint64 keys + qcnt[slot][lane]: queues parked
 //   between frames by the batched form.
 #pragma once
@@ -33,6 +37,7 @@ struct DevMap {
     const int4* nbr;
     const float2* cen;
     const float2* tri;  // 3 per triangle
+    const int4* rec;    // 3 per node: neighbours and their centroids
     // point location grid
     float gx0, gy0, inv_cs;
     float rx0, ry0, rx1, ry1;
@@ -79,8 +84,7 @@ struct DevSlots {
     long long* rounds;
     long long* expansions;
     long long* discoveries;
-    uint2* vis;             // P * V  {stamp, G << 1 | dir}
-    int* par;               // P * V  parent node
+    uint4* vis;             // P * V  {stamp, G << 1 | dir, parent, 0}
     unsigned long long* qsave;  // P * T * Q
     int* qcnt;              // P * T
 };

@@ -26,10 +26,13 @@
 // node id) in LDS, entry i of lane L at q[i*64+L] (bank-conflict free);
 // when full, a smaller key replaces the largest leaf and sifts up.  Claim
 // prefix sums and the first-meet choice are wave ballots + popcounts;
-// duplicate claims inside a round are found by a vectorised scan of a small
-// LDS candidate table.  Visited entries are 8 bytes {stamp, G<<1|dir} in a
-// per-slot table (stamp = search generation, so tables are never cleared);
-// parents go to a separate array read only by the corridor walk.
+// duplicate claims inside a round are found by an LDS claim hash.  Visited
+// entries are 16 bytes {stamp, G<<1|dir, parent, 0} in a per-slot table
+// (stamp = search generation, so tables are never cleared): a discovery is
+// one store to one cache line.  A node's neighbours and their centroids come
+// in one 48-byte record; the record and visited entry of the node the next
+// round will pop, and the visited entries of its neighbours, are requested
+// while the current round's frontier is pushed.
 //
 // Forms: plain (one search per segment, run to completion) and persistent
 // (segments take searches from an atomic ticket over the active list, run up
@@ -53,6 +56,11 @@ __device__ __forceinline__ unsigned long long make_key(float F, int node) {
 }
 
 __device__ __forceinline__ int key_node(unsigned long long k) { return (int)(unsigned)(k & 0xffffffffull); }
+
+// the probe half of a visited entry: {stamp, G << 1 | dir}
+__device__ __forceinline__ uint2 vis_head(const uint4* vis, int n) {
+    return *reinterpret_cast<const uint2*>(vis + n);
+}
 
 __device__ __forceinline__ float dist2(float2 a, float2 b) {
     float x = a.x - b.x;
@@ -188,12 +196,14 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
     int rounds_here = 0;
     long long rounds_tot = 0, exp_here = 0, disc_here = 0;
     int status = 0;
-    uint2* vis = slots.vis;
-    int* par = slots.par;
+    uint4* vis = slots.vis;
     // next round's pop, fetched while this round's frontier is pushed
     int pf_c = -1;
-    int4 pf_nb = make_int4(-1, -1, -1, -1);
+    bool pf_vn_ok = false;
+    int4 pf_r0 = make_int4(-1, -1, -1, -1), pf_r1 = make_int4(0, 0, 0, 0);
+    int2 pf_r2 = make_int2(0, 0);
     uint2 pf_vc = make_uint2(0u, 0u);
+    uint2 pf_vn[3] = {pf_vc, pf_vc, pf_vc};
 
     while (true) {
         // ---- acquire work for segments that have none
@@ -221,8 +231,8 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 exp_here = 0;
                 disc_here = 0;
                 vis = slots.vis + (size_t)slot * V;
-                par = slots.par + (size_t)slot * V;
                 pf_c = -1;
+                pf_vn_ok = false;
                 if (status == 3) {
                     // PathFinder.h:168-189
                     const float F0 = dist2(cS, cT) * sp.K;
@@ -230,12 +240,12 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                     if (l == 0) {
                         q[L] = make_key(F0, S);
                         cnt = 1;
-                        vis[S] = make_uint2(stamp, 1u);  // G 0, dir 1
+                        vis[S] = make_uint4(stamp, 1u, 0xffffffffu, 0u);  // G 0, dir 1
                     }
                     if (l == 1) {
                         q[L] = make_key(F0, Tg);
                         cnt = 1;
-                        vis[Tg] = make_uint2(stamp, 0u);  // G 0, dir 0
+                        vis[Tg] = make_uint4(stamp, 0u, 0xffffffffu, 0u);  // G 0, dir 0
                     }
                     status = 0;
                 } else {
@@ -254,19 +264,28 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
         const bool popped = has_req && cnt > 0;
         int c = -1, G = 0, cdir = 0;
         int nn[3] = {-1, -1, -1};
+        float2 cnb[3];
+        bool have_vn = false;
         if (popped) {
             c = key_node(heap_pop(q, L, cnt));
-            int4 nb = pf_nb;
+            int4 r0 = pf_r0, r1 = pf_r1;
+            int2 r2 = pf_r2;
             uint2 vc = pf_vc;
+            have_vn = pf_vn_ok && c == pf_c;
             if (c != pf_c) {
-                nb = map.nbr[c];
-                vc = vis[c];
+                r0 = map.rec[3 * (size_t)c];
+                r1 = map.rec[3 * (size_t)c + 1];
+                r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)c + 2);
+                vc = vis_head(vis, c);
             }
             G = (int)(vc.y >> 1);
             cdir = (int)(vc.y & 1u);
-            nn[0] = nb.x;
-            nn[1] = nn[0] >= 0 ? nb.y : -1;
-            nn[2] = nn[1] >= 0 ? nb.z : -1;
+            nn[0] = r0.x;
+            nn[1] = nn[0] >= 0 ? r0.y : -1;
+            nn[2] = nn[1] >= 0 ? r0.z : -1;
+            cnb[0] = make_float2(__int_as_float(r1.x), __int_as_float(r1.y));
+            cnb[1] = make_float2(__int_as_float(r1.z), __int_as_float(r1.w));
+            cnb[2] = make_float2(__int_as_float(r2.x), __int_as_float(r2.y));
         }
         float Fn[3];
         bool prior[3];
@@ -277,8 +296,8 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
             prior[s] = false;
             pdir[s] = 0;
             if (nn[s] >= 0) {
-                const float2 cn = map.cen[nn[s]];
-                const uint2 vn = vis[nn[s]];
+                const float2 cn = cnb[s];
+                const uint2 vn = have_vn ? pf_vn[s] : vis_head(vis, nn[s]);
                 const float h = cdir ? dist2(cn, cT) : dist2(cn, cS);
                 Fn[s] = (float)(G + 1) + h * sp.K;
                 prior[s] = (vn.x == stamp);
@@ -348,8 +367,7 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
             for (int s = 0; s < 3; ++s) {
                 if (claim[s]) {
                     stage[segbase + f] = make_key(Fn[s], nn[s]);
-                    vis[nn[s]] = make_uint2(stamp, ((unsigned)(G + 1) << 1) | (unsigned)cdir);
-                    par[nn[s]] = c;
+                    vis[nn[s]] = make_uint4(stamp, ((unsigned)(G + 1) << 1) | (unsigned)cdir, (unsigned)c, 0u);
                     ++f;
                 }
             }
@@ -384,11 +402,24 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 nt = k < nt ? k : nt;
             }
             pf_c = nt != ~0ull ? key_node(nt) : -1;
+            pf_vn_ok = false;
             if (pf_c >= 0) {
-                pf_nb = map.nbr[pf_c];
-                pf_vc = vis[pf_c];
+                pf_r0 = map.rec[3 * (size_t)pf_c];
+                pf_r1 = map.rec[3 * (size_t)pf_c + 1];
+                pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)pf_c + 2);
+                pf_vc = vis_head(vis, pf_c);
             }
             for (int f = l; f < tot; f += T) heap_push(q, L, cnt, Q, stage[segbase + f]);
+            // visited entries of the predicted node's neighbours: every write
+            // of this round is visible (barrier above) and none can happen
+            // before the next round reads them
+            if (pf_c >= 0) {
+                const int a0 = pf_r0.x, a1 = a0 >= 0 ? pf_r0.y : -1, a2 = a1 >= 0 ? pf_r0.z : -1;
+                if (a0 >= 0) pf_vn[0] = vis_head(vis, a0);
+                if (a1 >= 0) pf_vn[1] = vis_head(vis, a1);
+                if (a2 >= 0) pf_vn[2] = vis_head(vis, a2);
+                pf_vn_ok = true;
+            }
         }
         const unsigned long long nb = __ballot(cnt > 0) & smask;
         if (has_req) {
@@ -413,18 +444,18 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 // PathFinder.h:276-306: lane 0 writes the start side, lane 1 the target side
                 const int a = slots.agent[slot];
                 int* path = ag.path + (size_t)a * ag.path_cap;
-                const int lenS = (int)(vis[mS].y >> 1);
+                const int lenS = (int)(vis_head(vis, mS).y >> 1);
                 if (l == 0) {
                     int idx = lenS - 1, n = mS;
                     while (n != S) {
                         path[idx--] = n;
-                        n = par[n];
+                        n = (int)vis[n].z;
                     }
                 } else {
                     int idx = lenS, n = mT;
                     while (n != Tg) {
                         path[idx++] = n;
-                        n = par[n];
+                        n = (int)vis[n].z;
                     }
                     path[idx] = Tg;
                 }
@@ -440,7 +471,7 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 if (ended) {
                     const int a = slots.agent[slot];
                     if (status == 1) {  // PathFinder.h:280-290
-                        const int len = (int)(vis[mS].y >> 1) + (int)(vis[mT].y >> 1) + 1;
+                        const int len = (int)(vis_head(vis, mS).y >> 1) + (int)(vis_head(vis, mT).y >> 1) + 1;
                         slots.mS[slot] = mS;
                         slots.mT[slot] = mT;
                         ag.path_len[a] = len;

@@ -90,7 +90,7 @@ struct gmapf_server_s {
             if (V <= 0) throw ServeError(GMAPF_E_STATE, "map " + std::to_string(m) + " is not filled");
             const long long T = gc.path_finder_threads_per_agents, Q = gc.path_finder_queue_capacity;
             maps.push_back(h);
-            foot.push_back(T * Q * 8 + T * 4 + (long long)V * 12);
+            foot.push_back(T * Q * 8 + T * 4 + (long long)V * 16);
             slots.push_back(gc.path_finder_parallel_agents);
             live.push_back(0);
             busy.emplace_back();
