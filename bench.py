@@ -41,13 +41,17 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def tuned_triple(name, n):
+def tuned_entry(name, n):
     try:
         with open(TUNE_FILE) as f:
-            e = json.load(f)[f"{name}/{n}"]
-        return int(e["T"]), int(e["Q"]), int(e["budget"])
+            return json.load(f)[f"{name}/{n}"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def tuned_triple(name, n):
+    e = tuned_entry(name, n)
+    return (int(e["T"]), int(e["Q"]), int(e["budget"])) if e else None
 
 
 def measured_traffic(T, Q, budget, n):
@@ -335,6 +339,18 @@ def main():
         log("plain", {k: v for k, v in plain.items() if k not in ("stats", "launch")})
     bat = run_frames(sc, G.MODE_BATCHED, T, Q, B, args.warmup, args.steps, log=log)
     log("batched", {k: v for k, v in bat.items() if k not in ("stats", "launch")})
+    # the autotuner's fastest point, when it is not the reported one
+    fastest = None
+    te = tuned_entry(sc.name, sc.n) if tp and not (args.T or args.Q or args.budget) else None
+    fe = te.get("fastest") if te else None
+    if fe and (fe["T"], fe["Q"], fe["budget"]) != (T, Q, B):
+        fb = run_frames(sc, G.MODE_BATCHED, fe["T"], fe["Q"], fe["budget"], args.warmup, args.steps)
+        fp = None if args.skip_plain else run_frames(sc, G.MODE_PLAIN, fe["T"], fe["Q"], 0, args.warmup, args.steps)
+        fastest = dict(T=fe["T"], Q=fe["Q"], round_budget=fe["budget"], solves_per_s=round(fb["solves_per_s"], 1),
+                       p99_ms=round(fb["p99_ms"], 4),
+                       plain_solves_per_s=round(fp["solves_per_s"], 1) if fp else None,
+                       batched_over_plain=round(fb["solves_per_s"] / fp["solves_per_s"], 3) if fp else None)
+        log("fastest", fastest)
     roof = roofline_search(sc, T, Q, B)
     log("roofline", roof)
     cpu = None if args.skip_cpu else cpu_baseline(sc, loc, T, Q)
@@ -364,6 +380,9 @@ def main():
             "batched_over_plain": (round(bat["solves_per_s"] / plain["solves_per_s"], 3)
                                    if plain and plain["solves_per_s"] else None),
             "search_kernel_ms_per_launch": round(roof["ms_per_launch"], 4),
+            "autotune_choice": ("fastest point with batched >= 5x plain at p99 <= 4 ms"
+                                if te and te.get("target") else "fastest point at p99 <= 4 ms") if te else None,
+            "fastest": fastest,
             "parallelism": "single GPU",
         },
         "roofline": {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},

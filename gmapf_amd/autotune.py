@@ -2,15 +2,25 @@
 the per-frame round budget, chosen by measurement on the target workload.
 
 For every (T, Q) whose LDS footprint fits, continuous re-path runs of the
-batched form at each round budget of BUDGETS measure solves/s (found
-searches only: a small Q that fails fast does not count) and the p99 frame
-time over the benchmark's own window (5 warm-up + 120 frames: the mix of
-short and long searches in flight shifts towards long ones over the first
-hundred frames, so shorter probes favour small budgets).  The fastest (T, Q, budget) with p99 within the frame target wins after
-a longer confirmation run of the best few.  Results are cached per
-(workload, agents) in gmapf_amd/tuned.json.
+batched form at each round budget of BUDGETS, and one run of the plain form,
+measure solves/s (found searches only: a small Q that fails fast does not
+count) and the p99 frame time over the benchmark's own window (5 warm-up +
+120 frames: the mix of short and long searches in flight shifts towards long
+ones over the first hundred frames, so shorter probes favour small budgets).
+Plain and batched run the same (T, Q), so their corridors are identical.
 
-    python -m gmapf_amd.autotune [--agents 16384] [--target-ms 4.0]
+Two points are kept:
+  * "target": the highest batched solves/s whose p99 frame is within
+    --target-ms and whose solves/s is at least --ratio times the plain
+    step's at the same (T, Q) (the batching goal);
+  * "fastest": the highest batched solves/s within --target-ms, whatever
+    the ratio.
+The best few candidates of each are re-measured before the choice.  Results
+are cached per (workload, agents) in gmapf_amd/tuned.json; the benchmark
+reports the "target" point (or "fastest" when no point reaches the ratio)
+and measures the other beside it.
+
+    python -m gmapf_amd.autotune [--agents 16384] [--target-ms 4.0] [--ratio 5.0]
 """
 from __future__ import annotations
 
@@ -20,7 +30,7 @@ import os
 import sys
 import time
 
-from . import MODE_BATCHED, GmapfError
+from . import MODE_BATCHED, MODE_PLAIN, GmapfError
 from . import scenarios as S
 from .runner import Locators, run_frames
 
@@ -28,8 +38,6 @@ TUNE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json
 T_SET = (2, 4, 8, 16, 32, 64)
 Q_SET = (16, 32, 64, 128, 256)
 LDS_BYTES = 160 * 1024
-
-
 BUDGETS = (48, 64, 96, 128, 192, 256)
 
 
@@ -37,40 +45,64 @@ def lds_fits(Q):
     return 64 * Q * 8 + 4096 <= LDS_BYTES
 
 
-def tune(sc, target_ms=4.0, budgets=BUDGETS, warmup=5, steps=120, confirm_steps=120, top=3, log=print):
+def _confirm(sc, rows, warmup, steps, log):
+    for r in rows:
+        if "solves_confirm" in r:
+            continue
+        c = run_frames(sc, MODE_BATCHED, r["T"], r["Q"], r["budget"], warmup, steps)
+        r["solves_confirm"] = c["solves_per_s"]
+        r["p99_confirm"] = c["p99_ms"]
+        log(f"[tune] confirm T={r['T']} Q={r['Q']} B={r['budget']}: {c['solves_per_s']:.4g} solves/s "
+            f"p99 {c['p99_ms']:.2f} ms (plain {r['plain_solves_per_s']:.4g})")
+
+
+def tune(sc, target_ms=4.0, ratio=5.0, budgets=BUDGETS, warmup=5, steps=120, top=3, log=print):
     table = []
     for T in T_SET:
         for Q in Q_SET:
             if not lds_fits(Q):
                 continue
+            try:
+                p = run_frames(sc, MODE_PLAIN, T, Q, 0, warmup, steps)
+            except GmapfError as e:  # does not fit this device
+                log(f"[tune] T={T} Q={Q}: {e}")
+                continue
+            log(f"[tune] T={T:2d} Q={Q:3d} plain: {p['solves_per_s']:.4g} solves/s p99 {p['p99_ms']:.2f} ms")
             for B in budgets:
                 t0 = time.time()
-                try:
-                    r = run_frames(sc, MODE_BATCHED, T, Q, B, warmup, steps)
-                except GmapfError as e:  # does not fit this device
-                    log(f"[tune] T={T} Q={Q}: {e}")
-                    break
+                r = run_frames(sc, MODE_BATCHED, T, Q, B, warmup, steps)
                 row = dict(T=T, Q=Q, budget=B, solves_per_s=r["solves_per_s"], p50_ms=r["p50_ms"],
-                           p99_ms=r["p99_ms"], found=r["found"], secs=round(time.time() - t0, 1))
+                           p99_ms=r["p99_ms"], found=r["found"], plain_solves_per_s=p["solves_per_s"],
+                           plain_p99_ms=p["p99_ms"], ratio=r["solves_per_s"] / max(p["solves_per_s"], 1e-9),
+                           secs=round(time.time() - t0, 1))
                 table.append(row)
                 log(f"[tune] T={T:2d} Q={Q:3d} B={B:3d}: {r['solves_per_s']:.4g} solves/s "
-                    f"p50 {r['p50_ms']:.2f} p99 {r['p99_ms']:.2f} ms")
+                    f"p50 {r['p50_ms']:.2f} p99 {r['p99_ms']:.2f} ms ratio {row['ratio']:.2f}")
                 if r["p50_ms"] > target_ms:  # larger budgets only make frames longer
                     break
-    ok = [r for r in table if r["p99_ms"] <= target_ms] or sorted(table, key=lambda r: r["p99_ms"])[:top]
-    ranked = sorted(ok, key=lambda r: -r["solves_per_s"])[:top]
-    best = None
-    for r in ranked:
-        c = run_frames(sc, MODE_BATCHED, r["T"], r["Q"], r["budget"], warmup, confirm_steps)
-        r["solves_confirm"] = c["solves_per_s"]
-        r["p99_confirm"] = c["p99_ms"]
-        log(f"[tune] confirm T={r['T']} Q={r['Q']} B={r['budget']}: {c['solves_per_s']:.4g} solves/s "
-            f"p99 {c['p99_ms']:.2f} ms")
-        if c["p99_ms"] <= target_ms and (best is None or c["solves_per_s"] > best["solves_confirm"]):
-            best = r
-    if best is None:
-        best = min(ranked, key=lambda r: r["p99_confirm"])
-    return best, table
+
+    def pick(cands, need):
+        ranked = sorted(cands, key=lambda r: -r["solves_per_s"])[:top]
+        _confirm(sc, ranked, warmup, steps, log)
+        ok = [r for r in ranked
+              if r["p99_confirm"] <= target_ms and r["solves_confirm"] >= need * r["plain_solves_per_s"]]
+        return max(ok, key=lambda r: r["solves_confirm"]) if ok else None
+
+    within = [r for r in table if r["p99_ms"] <= target_ms]
+    with_ratio = [r for r in within if r["ratio"] >= ratio]
+    fastest = pick(within, 0.0) if within else None
+    target = pick(with_ratio, ratio) if with_ratio else None
+    if fastest is None:
+        fastest = min(table, key=lambda r: r["p99_ms"])
+    return target, fastest, table
+
+
+def _entry(r):
+    if r is None:
+        return None
+    return dict(T=r["T"], Q=r["Q"], budget=r["budget"], solves_per_s=r.get("solves_confirm", r["solves_per_s"]),
+                p99_ms=r.get("p99_confirm", r["p99_ms"]), plain_solves_per_s=r["plain_solves_per_s"],
+                ratio=r.get("solves_confirm", r["solves_per_s"]) / max(r["plain_solves_per_s"], 1e-9))
 
 
 def main():
@@ -78,6 +110,7 @@ def main():
     ap.add_argument("--agents", type=int, default=16384)
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--target-ms", type=float, default=4.0)
+    ap.add_argument("--ratio", type=float, default=5.0)
     ap.add_argument("--out", default=TUNE_FILE)
     args = ap.parse_args()
     loc = Locators()
@@ -86,18 +119,19 @@ def main():
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
 
-    best, table = tune(sc, args.target_ms, log=log)
+    target, fastest, table = tune(sc, args.target_ms, args.ratio, log=log)
     try:
         with open(args.out) as f:
             d = json.load(f)
     except (OSError, ValueError):
         d = {}
-    d[f"{sc.name}/{sc.n}"] = dict(T=best["T"], Q=best["Q"], budget=best["budget"],
-                                 solves_per_s=best.get("solves_confirm"), p99_ms=best.get("p99_confirm"),
-                                 target_ms=args.target_ms, table=table)
+    chosen = target or fastest
+    d[f"{sc.name}/{sc.n}"] = dict(T=chosen["T"], Q=chosen["Q"], budget=chosen["budget"],
+                                 target=_entry(target), fastest=_entry(fastest), target_ms=args.target_ms,
+                                 ratio_goal=args.ratio, table=table)
     with open(args.out, "w") as f:
         json.dump(d, f, indent=1)
-    print(json.dumps({k: best[k] for k in ("T", "Q", "budget")}))
+    print(json.dumps({"target": _entry(target), "fastest": _entry(fastest)}))
 
 
 if __name__ == "__main__":

@@ -135,6 +135,19 @@ __device__ __forceinline__ unsigned long long heap_pop(unsigned long long* q, in
     return top;
 }
 
+// Ordering point between the lanes of the workgroup.  A workgroup is one
+// wave (launch_search launches kWave threads), and the memory operations of
+// one wavefront are performed in order with respect to each other (LDS and
+// the CU's vector L1 alike), so a wavefront-scope fence plus a wave barrier
+// (both compiler-level) orders the lanes' LDS and global accesses without
+// the wait for outstanding loads and stores that a workgroup barrier
+// implies.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int T>
 __device__ __forceinline__ unsigned long long seg_mask(int sub) {
     if constexpr (T == 64) {
@@ -258,7 +271,7 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
             }
         }
         if (__ballot(has_req) == 0ull) break;
-        __syncthreads();  // seeds, restored queues and visited writes visible
+        wave_sync();  // seeds, restored queues and visited writes visible
 
         // ---- pop + neighbour fetch
         const bool popped = has_req && cnt > 0;
@@ -317,7 +330,7 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
         if (l == T - 1 && CS > 3 * T) {
             for (int k = 3 * T; k < CS; ++k) cand[cbase + k] = -1;
         }
-        __syncthreads();
+        wave_sync();
 
         // ---- duplicate claims of this round: first earlier candidate with the
         // same node, from the hash; a slot taken by another node falls back to
@@ -385,7 +398,7 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
             mT = __shfl(b, w, kWave);
         }
         const unsigned long long pb = __ballot(popped) & smask;
-        __syncthreads();
+        wave_sync();
         // empty this round's claim-hash slots for the next round
 #pragma unroll
         for (int s = 0; s < 3; ++s)
