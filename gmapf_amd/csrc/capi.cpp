@@ -347,13 +347,15 @@ struct gmapf_s {
         dsl.rounds = dalloc<long long>(P, "slot rounds");
         dsl.expansions = dalloc<long long>(P, "slot expansions");
         dsl.discoveries = dalloc<long long>(P, "slot discoveries");
-        dsl.vis = dalloc<uint4>((size_t)P * V, "visited tables");
+        dsl.VP = ((long long)V + 63) / 64 * 64;
+        dsl.mark = dalloc<unsigned char>((size_t)P * dsl.VP, "visited marks");
+        dsl.vis = dalloc<uint2>((size_t)P * V, "visited entries");
         dsl.qsave = dalloc<unsigned long long>((size_t)P * T * Q, "parked queues");
         dsl.qcnt = dalloc<int>((size_t)P * T, "parked queue counts");
         HIPCHK(launch_fill(dsl.agent, -1, P, s1));
         HIPCHK(launch_fill(dsl.status, 0, P, s1));
         HIPCHK(hipMemsetAsync(dsl.stamp, 0, P * sizeof(unsigned), s1));
-        HIPCHK(hipMemsetAsync(dsl.vis, 0, (size_t)P * V * sizeof(uint4), s1));
+        HIPCHK(hipMemsetAsync(dsl.mark, 0, (size_t)P * dsl.VP, s1));
         HIPCHK(hipMemsetAsync(dsl.rounds, 0, P * sizeof(long long), s1));
         HIPCHK(hipMemsetAsync(dsl.expansions, 0, P * sizeof(long long), s1));
         HIPCHK(hipMemsetAsync(dsl.discoveries, 0, P * sizeof(long long), s1));
@@ -372,7 +374,8 @@ struct gmapf_s {
         dfr.adm_agent = dalloc<int>(P, "admitted agents");
         dfr.active = dalloc<int>(P, "active slots");
         dfr.ended = dalloc<int>(P, "ended slots");
-        dfr.counters = dalloc<int>(8, "counters");
+        dfr.counters = dalloc<int>(16, "counters");
+        dfr.clear_list = dalloc<int>(P, "mark clear list");
         dfr.stats = dalloc<unsigned long long>(8, "stats");
         dfr.ncell = dmap.hnx * dmap.hny;
         dfr.hkey = dalloc<int>(Nn, "hash keys");
@@ -383,7 +386,7 @@ struct gmapf_s {
         dfr.horder = dalloc<int>(Nn, "hash order");
         dfr.hblk = dalloc<int>(dfr.ncell / scan_block + 2, "hash scan partials");
         dfr.stepv = dalloc<float>(1, "step");
-        HIPCHK(launch_fill(dfr.counters, 0, 8, s1));
+        HIPCHK(launch_fill(dfr.counters, 0, 16, s1));
         HIPCHK(launch_fill_u64(dfr.stats, 0, 8, s1));
         HIPCHK(launch_fill(dfr.hkey, -1, Nn, s1));
 

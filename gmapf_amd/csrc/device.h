@@ -18,11 +18,15 @@
 //   path[id * path_cap ...] corridor, gen int (bumped by SetAgentTargPos),
 //   inflight int (search slot or -1), in_m int (moving set of this frame).
 // Search slots (P of them, PathFinderParallelAgents):
-//   vis[slot * V + node] uint4 {stamp, G << 1 | dir, parent, 0}: one aligned
-//   16-byte entry, so a discovery is one store to one cache line; an entry
-//   belongs to the current search of the slot only when its stamp equals
-//   slot_stamp[slot], so slots are never cleared (replaces
-//   PathFinder.h:119-136).
+//   mark[slot * VP + node] byte (m << 1) | dir, m = stamp % 127 + 1 of the
+//   search that claimed the node: the probe a search makes for every
+//   neighbour reads one byte (64 nodes per cache line); a node belongs to
+//   the current search only when its m matches, so marks are cleared only
+//   when m wraps (once every 127 searches of a slot), and tables are never
+//   cleared per search (replaces PathFinder.h:119-136).
+//   vis[slot * V + node] uint2 {G << 1 | dir, parent}: read for the node a
+//   lane pops and by the corridor walk, i.e. only for nodes of the current
+//   search, so it needs no tag.
 //   qsave[slot][lane][Q] uint64 keys + qcnt[slot][lane]: queues parked
 //   between frames by the batched form.
 #pragma once
@@ -84,7 +88,9 @@ struct DevSlots {
     long long* rounds;
     long long* expansions;
     long long* discoveries;
-    uint4* vis;             // P * V  {stamp, G << 1 | dir, parent, 0}
+    unsigned char* mark;    // P * VP  (m << 1) | dir
+    uint2* vis;             // P * V   {G << 1 | dir, parent}
+    long long VP;           // mark stride: V rounded up to 64
     unsigned long long* qsave;  // P * T * Q
     int* qcnt;              // P * T
 };
@@ -105,6 +111,8 @@ struct DevFrame {
     int* counters;   // [0] n_requests [1] n_active [2] work ticket [3] n_ended
                      // [4] n_free [5] moving count [6] n_admitted
                      // [7] blocks done in the admission count pass
+                     // [8] slots whose marks wrap this frame
+    int* clear_list; // P: those slots
     unsigned long long* stats;  // [0] found [1] fail [2] expansions [3] discoveries
                                 // [4] corridor nodes [5] rounds [6] admitted
     // spatial hash

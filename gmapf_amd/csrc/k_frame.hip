@@ -296,6 +296,7 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_count(DevAgents ag, Dev
         fr.counters[5] = moving;
         fr.counters[6] = admitted;
         fr.counters[7] = 0;  // every block has counted itself: ready for the next frame
+        fr.counters[8] = 0;
         fr.stats[6] += (unsigned long long)admitted;
     }
 }
@@ -333,7 +334,9 @@ __global__ void k_admit_slots(DevAgents ag, DevSlots sl, DevFrame fr) {
     sl.gen[s] = ag.gen[i];
     sl.S[s] = ag.curr[i];
     sl.Tg[s] = ag.targ[i];
-    sl.stamp[s] = sl.stamp[s] + 1u;
+    const unsigned st = sl.stamp[s] + 1u;
+    sl.stamp[s] = st;
+    if (st % 127u == 0u) fr.clear_list[atomicAdd(&fr.counters[8], 1)] = s;  // mark value wraps to 1
     sl.status[s] = 3;
     sl.rounds[s] = 0;
     sl.expansions[s] = 0;
@@ -341,6 +344,16 @@ __global__ void k_admit_slots(DevAgents ag, DevSlots sl, DevFrame fr) {
     ag.inflight[i] = s;
     ag.path_len[i] = 0;  // old corridor released (PathFinder.h:104-108)
     fr.active[fr.counters[1] - admitted + k] = s;
+}
+
+// marks of the slots whose mark value wrapped: zero before their search seeds
+__global__ void k_clear_marks(DevSlots sl, DevFrame fr) {
+    const int n = fr.counters[8];
+    const long long words = sl.VP / 16;
+    for (int e = blockIdx.x; e < n; e += gridDim.x) {
+        uint4* m = reinterpret_cast<uint4*>(sl.mark + (size_t)fr.clear_list[e] * sl.VP);
+        for (long long w = threadIdx.x; w < words; w += blockDim.x) m[w] = make_uint4(0u, 0u, 0u, 0u);
+    }
 }
 
 // GMAPF.cu:247-250 for target changes made after start, read from pinned
@@ -552,6 +565,7 @@ hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& 
     hipLaunchKernelGGL(k_admit_count, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
     hipLaunchKernelGGL(k_admit_scatter, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
     hipLaunchKernelGGL(k_admit_slots, dim3(nblk(sl.P, 256)), dim3(256), 0, st, ag, sl, fr);
+    hipLaunchKernelGGL(k_clear_marks, dim3(256), dim3(256), 0, st, sl, fr);
     return hipGetLastError();
 }
 

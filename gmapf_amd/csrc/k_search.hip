@@ -27,12 +27,14 @@
 // when full, a smaller key replaces the largest leaf and sifts up.  Claim
 // prefix sums and the first-meet choice are wave ballots + popcounts;
 // duplicate claims inside a round are found by an LDS claim hash.  Visited
-// entries are 16 bytes {stamp, G<<1|dir, parent, 0} in a per-slot table
-// (stamp = search generation, so tables are never cleared): a discovery is
-// one store to one cache line.  A node's neighbours and their centroids come
-// in one 48-byte record; the record and visited entry of the node the next
-// round will pop, and the visited entries of its neighbours, are requested
-// while the current round's frontier is pushed.
+// state is split by use: a per-slot byte map (m << 1 | dir, m = the search's
+// 7-bit generation, so maps are not cleared per search) answers the probe of
+// every neighbour with one byte, 64 nodes per cache line; an 8-byte entry
+// {G << 1 | dir, parent} is read only for popped nodes and by the corridor
+// walk.  A node's neighbours and their centroids come in one 48-byte record;
+// the record and entry of the node the next round will pop, and the marks of
+// its neighbours, are requested while the current round's frontier is
+// pushed.
 //
 // Forms: plain (one search per segment, run to completion) and persistent
 // (segments take searches from an atomic ticket over the active list, run up
@@ -56,11 +58,6 @@ __device__ __forceinline__ unsigned long long make_key(float F, int node) {
 }
 
 __device__ __forceinline__ int key_node(unsigned long long k) { return (int)(unsigned)(k & 0xffffffffull); }
-
-// the probe half of a visited entry: {stamp, G << 1 | dir}
-__device__ __forceinline__ uint2 vis_head(const uint4* vis, int n) {
-    return *reinterpret_cast<const uint2*>(vis + n);
-}
 
 __device__ __forceinline__ float dist2(float2 a, float2 b) {
     float x = a.x - b.x;
@@ -204,19 +201,20 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
     int slot = -1;
     bool has_req = false, no_more = false, took_plain = false;
     int S = 0, Tg = 0, cnt = 0;
-    unsigned stamp = 0;
+    unsigned mval = 0;  // this search's mark generation, 1..127
     float2 cS = make_float2(0.f, 0.f), cT = make_float2(0.f, 0.f);
     int rounds_here = 0;
     long long rounds_tot = 0, exp_here = 0, disc_here = 0;
     int status = 0;
-    uint4* vis = slots.vis;
+    uint2* vis = slots.vis;
+    unsigned char* mark = slots.mark;
     // next round's pop, fetched while this round's frontier is pushed
     int pf_c = -1;
     bool pf_vn_ok = false;
     int4 pf_r0 = make_int4(-1, -1, -1, -1), pf_r1 = make_int4(0, 0, 0, 0);
     int2 pf_r2 = make_int2(0, 0);
     uint2 pf_vc = make_uint2(0u, 0u);
-    uint2 pf_vn[3] = {pf_vc, pf_vc, pf_vc};
+    unsigned pf_vn[3] = {0u, 0u, 0u};
 
     while (true) {
         // ---- acquire work for segments that have none
@@ -235,7 +233,7 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 slot = fr.active[item];
                 S = slots.S[slot];
                 Tg = slots.Tg[slot];
-                stamp = slots.stamp[slot];
+                mval = slots.stamp[slot] % 127u + 1u;
                 status = slots.status[slot];
                 rounds_tot = slots.rounds[slot];
                 cS = map.cen[S];
@@ -244,6 +242,7 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 exp_here = 0;
                 disc_here = 0;
                 vis = slots.vis + (size_t)slot * V;
+                mark = slots.mark + (size_t)slot * slots.VP;
                 pf_c = -1;
                 pf_vn_ok = false;
                 if (status == 3) {
@@ -253,12 +252,14 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                     if (l == 0) {
                         q[L] = make_key(F0, S);
                         cnt = 1;
-                        vis[S] = make_uint4(stamp, 1u, 0xffffffffu, 0u);  // G 0, dir 1
+                        vis[S] = make_uint2(1u, 0xffffffffu);  // G 0, dir 1
+                        mark[S] = (unsigned char)((mval << 1) | 1u);
                     }
                     if (l == 1) {
                         q[L] = make_key(F0, Tg);
                         cnt = 1;
-                        vis[Tg] = make_uint4(stamp, 0u, 0xffffffffu, 0u);  // G 0, dir 0
+                        vis[Tg] = make_uint2(0u, 0xffffffffu);  // G 0, dir 0
+                        mark[Tg] = (unsigned char)(mval << 1);
                     }
                     status = 0;
                 } else {
@@ -289,10 +290,10 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 r0 = map.rec[3 * (size_t)c];
                 r1 = map.rec[3 * (size_t)c + 1];
                 r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)c + 2);
-                vc = vis_head(vis, c);
+                vc = vis[c];
             }
-            G = (int)(vc.y >> 1);
-            cdir = (int)(vc.y & 1u);
+            G = (int)(vc.x >> 1);
+            cdir = (int)(vc.x & 1u);
             nn[0] = r0.x;
             nn[1] = nn[0] >= 0 ? r0.y : -1;
             nn[2] = nn[1] >= 0 ? r0.z : -1;
@@ -310,11 +311,11 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
             pdir[s] = 0;
             if (nn[s] >= 0) {
                 const float2 cn = cnb[s];
-                const uint2 vn = have_vn ? pf_vn[s] : vis_head(vis, nn[s]);
+                const unsigned mk = have_vn ? pf_vn[s] : (unsigned)mark[nn[s]];
                 const float h = cdir ? dist2(cn, cT) : dist2(cn, cS);
                 Fn[s] = (float)(G + 1) + h * sp.K;
-                prior[s] = (vn.x == stamp);
-                pdir[s] = (int)(vn.y & 1u);
+                prior[s] = (mk >> 1) == mval;
+                pdir[s] = (int)(mk & 1u);
             }
         }
         int* hseg = &htab[sub * HS];
@@ -380,7 +381,8 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
             for (int s = 0; s < 3; ++s) {
                 if (claim[s]) {
                     stage[segbase + f] = make_key(Fn[s], nn[s]);
-                    vis[nn[s]] = make_uint4(stamp, ((unsigned)(G + 1) << 1) | (unsigned)cdir, (unsigned)c, 0u);
+                    mark[nn[s]] = (unsigned char)((mval << 1) | (unsigned)cdir);
+                    vis[nn[s]] = make_uint2(((unsigned)(G + 1) << 1) | (unsigned)cdir, (unsigned)c);
                     ++f;
                 }
             }
@@ -420,7 +422,7 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 pf_r0 = map.rec[3 * (size_t)pf_c];
                 pf_r1 = map.rec[3 * (size_t)pf_c + 1];
                 pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)pf_c + 2);
-                pf_vc = vis_head(vis, pf_c);
+                pf_vc = vis[pf_c];
             }
             for (int f = l; f < tot; f += T) heap_push(q, L, cnt, Q, stage[segbase + f]);
             // visited entries of the predicted node's neighbours: every write
@@ -428,9 +430,9 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
             // before the next round reads them
             if (pf_c >= 0) {
                 const int a0 = pf_r0.x, a1 = a0 >= 0 ? pf_r0.y : -1, a2 = a1 >= 0 ? pf_r0.z : -1;
-                if (a0 >= 0) pf_vn[0] = vis_head(vis, a0);
-                if (a1 >= 0) pf_vn[1] = vis_head(vis, a1);
-                if (a2 >= 0) pf_vn[2] = vis_head(vis, a2);
+                if (a0 >= 0) pf_vn[0] = mark[a0];
+                if (a1 >= 0) pf_vn[1] = mark[a1];
+                if (a2 >= 0) pf_vn[2] = mark[a2];
                 pf_vn_ok = true;
             }
         }
@@ -457,18 +459,18 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 // PathFinder.h:276-306: lane 0 writes the start side, lane 1 the target side
                 const int a = slots.agent[slot];
                 int* path = ag.path + (size_t)a * ag.path_cap;
-                const int lenS = (int)(vis_head(vis, mS).y >> 1);
+                const int lenS = (int)(vis[mS].x >> 1);
                 if (l == 0) {
                     int idx = lenS - 1, n = mS;
                     while (n != S) {
                         path[idx--] = n;
-                        n = (int)vis[n].z;
+                        n = (int)vis[n].y;
                     }
                 } else {
                     int idx = lenS, n = mT;
                     while (n != Tg) {
                         path[idx++] = n;
-                        n = (int)vis[n].z;
+                        n = (int)vis[n].y;
                     }
                     path[idx] = Tg;
                 }
@@ -484,7 +486,7 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 if (ended) {
                     const int a = slots.agent[slot];
                     if (status == 1) {  // PathFinder.h:280-290
-                        const int len = (int)(vis_head(vis, mS).y >> 1) + (int)(vis_head(vis, mT).y >> 1) + 1;
+                        const int len = (int)(vis[mS].x >> 1) + (int)(vis[mT].x >> 1) + 1;
                         slots.mS[slot] = mS;
                         slots.mT[slot] = mT;
                         ag.path_len[a] = len;

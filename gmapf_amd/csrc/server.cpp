@@ -90,7 +90,8 @@ struct gmapf_server_s {
             if (V <= 0) throw ServeError(GMAPF_E_STATE, "map " + std::to_string(m) + " is not filled");
             const long long T = gc.path_finder_threads_per_agents, Q = gc.path_finder_queue_capacity;
             maps.push_back(h);
-            foot.push_back(T * Q * 8 + T * 4 + (long long)V * 16);
+            const long long VP = ((long long)V + 63) / 64 * 64;
+            foot.push_back(T * Q * 8 + T * 4 + VP + (long long)V * 8);
             slots.push_back(gc.path_finder_parallel_agents);
             live.push_back(0);
             busy.emplace_back();

@@ -21,8 +21,9 @@
  * back by the memory budget stops the scan, so a large-map request is never
  * overtaken indefinitely by small ones.  Footprint of a request on a map with
  * V nodes, T lanes and queue capacity Q:
- *   T * Q * 8 (parked queue keys) + T * 4 (queue counts) + V * 16 (visited
- *   table: stamp, G and direction, parent),
+ *   T * Q * 8 (parked queue keys) + T * 4 (queue counts) + VP (visited
+ *   marks, one byte per node, V rounded up to 64) + V * 8 (G, direction and
+ *   parent per node),
  * i.e. the device memory its search slot keeps busy while it runs.  An
  * admitted request becomes the agent's SetAgentTargPos of the next frame
  * (reference include/gmapf/GMAPF.h:44); it is complete once the agent leaves

@@ -61,7 +61,7 @@ G_E_ARG, G_E_POS, G_E_FULL = -1, -3, -7
 def test_footprint_formula():
     g, _, V = _engine(T=16, Q=64)
     s = Server([g])
-    assert s.footprint(0) == 16 * 64 * 8 + 16 * 4 + V * 16
+    assert s.footprint(0) == 16 * 64 * 8 + 16 * 4 + (V + 63) // 64 * 64 + V * 8
 
 
 def test_plan_respects_slots_and_serialises_agents():

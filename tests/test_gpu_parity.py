@@ -132,3 +132,43 @@ def test_device_locate_matches_host_and_oracle(oracle):
     brute = oracle.OracleMap(tri, nbr).locate(pts, grid=False)
     assert np.array_equal(host, brute)
     assert np.array_equal(dev, brute)
+
+
+@pytest.mark.parametrize("mode,budget", [(G.MODE_PLAIN, 0), (G.MODE_BATCHED, 2)])
+def test_one_slot_many_searches_mark_wrap(oracle, mode, budget):
+    """400 consecutive searches through one slot: the 7-bit visited-mark
+    generation wraps three times (the slot's marks are cleared on each wrap);
+    every corridor still equals the oracle's search from the agent's node."""
+    tri, nbr = H.grid_map(17, 20, 16, cell=4.0, p=0.2)
+    cen = S.centroids(tri)
+    lab = S.components(nbr)
+    big = np.bincount(lab).argmax()
+    nodes = np.nonzero(lab == big)[0]
+    r = S.stream(23, 5)
+    cfg = G.Config(agents_max_count=3, path_finder_parallel_agents=1, path_finder_threads_per_agents=4,
+                   path_finder_queue_capacity=8, agent_radius=0.25, agent_speed=1.0, step_mode=mode,
+                   round_budget=max(1, budget))
+    g = H.make_gpu(tri, nbr, cfg, [cen[nodes[k]] for k in range(3)])
+    om = oracle.OracleMap(tri, nbr)
+    print(f"mark wrap mode={mode} V={len(tri)}", flush=True)
+    searches = found = 0
+    while searches < 400:
+        a = searches % 3
+        _, _, cur = g.agents(with_nodes=True)
+        tg = int(nodes[r.randint(len(nodes))])
+        if tg == cur[a]:
+            continue
+        g.set_agent_targ_pos(a, *cen[tg])
+        for _ in range(200):
+            g.step(1.0 / 60)
+            if g.get_agent_state(a) != G.SEARCH:
+                break
+        st = g.get_agent_state(a)
+        assert st != G.SEARCH
+        ok, cor, _ = om.search(int(cur[a]), tg, 4, 8, 1.0)
+        assert (st == G.MOVE) == ok, f"search {searches}"
+        if ok:
+            found += 1
+            assert np.array_equal(g.corridor(a), cor), f"search {searches}"
+        searches += 1
+    assert found > 300

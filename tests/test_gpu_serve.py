@@ -55,7 +55,7 @@ def test_results_alone_equal_results_in_the_mix(oracle):
     locs = Locators()
     smaps = _maps(locs)
     engines = [build_map(sm, T, Q, round_budget=3) for sm in smaps]
-    fb = [e.launch_info()["V"] * 16 + T * Q * 8 + T * 4 for e in engines]
+    fb = [(e.launch_info()["V"] + 63) // 64 * 64 + e.launch_info()["V"] * 8 + T * Q * 8 + T * 4 for e in engines]
     budget = 12 * fb[1] + 12 * fb[0]  # fewer than the slots: the budget binds
     srv = Server(engines, queue_memory_budget=budget, record_start_nodes=True)
     assert [srv.footprint(m) for m in range(2)] == fb
