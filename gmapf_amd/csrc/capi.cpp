@@ -342,11 +342,6 @@ struct gmapf_s {
         dsl.Tg = dalloc<int>(P, "slot target");
         dsl.stamp = dalloc<unsigned>(P, "slot stamp");
         dsl.status = dalloc<int>(P, "slot status");
-        dsl.mS = dalloc<int>(P, "slot mS");
-        dsl.mT = dalloc<int>(P, "slot mT");
-        dsl.rounds = dalloc<long long>(P, "slot rounds");
-        dsl.expansions = dalloc<long long>(P, "slot expansions");
-        dsl.discoveries = dalloc<long long>(P, "slot discoveries");
         dsl.VP = ((long long)V + 63) / 64 * 64;
         dsl.mark = dalloc<unsigned char>((size_t)P * dsl.VP, "visited marks");
         dsl.vis = dalloc<uint2>((size_t)P * V, "visited entries");
@@ -356,9 +351,6 @@ struct gmapf_s {
         HIPCHK(launch_fill(dsl.status, 0, P, s1));
         HIPCHK(hipMemsetAsync(dsl.stamp, 0, P * sizeof(unsigned), s1));
         HIPCHK(hipMemsetAsync(dsl.mark, 0, (size_t)P * dsl.VP, s1));
-        HIPCHK(hipMemsetAsync(dsl.rounds, 0, P * sizeof(long long), s1));
-        HIPCHK(hipMemsetAsync(dsl.expansions, 0, P * sizeof(long long), s1));
-        HIPCHK(hipMemsetAsync(dsl.discoveries, 0, P * sizeof(long long), s1));
 
         // ---- frame scratch
         dfr = DevFrame{};

@@ -83,11 +83,6 @@ struct DevSlots {
     int* Tg;
     unsigned* stamp;
     int* status;    // 0 running, 1 found, 2 exhausted, 3 fresh (needs seeding)
-    int* mS;
-    int* mT;
-    long long* rounds;
-    long long* expansions;
-    long long* discoveries;
     unsigned char* mark;    // P * VP  (m << 1) | dir
     uint2* vis;             // P * V   {G << 1 | dir, parent}
     long long VP;           // mark stride: V rounded up to 64

@@ -338,9 +338,6 @@ __global__ void k_admit_slots(DevAgents ag, DevSlots sl, DevFrame fr) {
     sl.stamp[s] = st;
     if (st % 127u == 0u) fr.clear_list[atomicAdd(&fr.counters[8], 1)] = s;  // mark value wraps to 1
     sl.status[s] = 3;
-    sl.rounds[s] = 0;
-    sl.expansions[s] = 0;
-    sl.discoveries[s] = 0;
     ag.inflight[i] = s;
     ag.path_len[i] = 0;  // old corridor released (PathFinder.h:104-108)
     fr.active[fr.counters[1] - admitted + k] = s;

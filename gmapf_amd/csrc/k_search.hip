@@ -154,43 +154,96 @@ __device__ __forceinline__ unsigned long long seg_mask(int sub) {
     }
 }
 
-// candidate table: 3T entries per segment, padded to a multiple of 4
-template <int T>
-constexpr int cand_stride() {
-    return ((3 * T + 3) / 4) * 4;
-}
-
-// per-segment claim hash: power of two >= 6T slots of (node << 8 | k)
+// per-segment claim table: open addressing, power of two >= 16T slots for the
+// at most 3T candidates of a round (load factor <= 3/16)
 template <int T>
 constexpr int hash_bits() {
     int b = 1;
-    while ((1 << b) < 6 * T) ++b;
+    while ((1 << b) < 16 * T) ++b;
     return b;
 }
-constexpr int kHashEmpty = 0x7fffffff;
+constexpr int kHashEmpty = -1;
+constexpr int kHashNone = 0x7fffffff;
 
 __device__ __forceinline__ unsigned claim_hash(int n, int bits) {
     return ((unsigned)n * 2654435761u) >> (32 - bits);
 }
 
+// Corridor walk of one side of a found search (PathFinder.h:276-306), one
+// parent step per call so the walk of a search that ended overlaps the rounds
+// of the other searches of the workgroup.  Lane 0 of the segment walks the
+// start side from mS down to the start node (exclusive), writing indices
+// G(mS)-1 .. 0; lane 1 walks the target side from mT to the target node
+// (inclusive), writing indices G(mS) .. G(mS)+G(mT).  `ent` holds the visited
+// entry of `n`, requested by the previous step; lane 1 also requested mS's
+// entry, whose first word (`gS`) gives the start side's length.
+struct Walk {
+    bool active = false;
+    bool first = false;
+    int n = 0;
+    int idx = 0;
+    int term = 0;
+    int agent = 0;
+    int slot = 0;
+    uint2 ent = make_uint2(0u, 0u);
+    unsigned gS = 0u;
+};
+
+__device__ __forceinline__ void walk_step(Walk& w, int side, const DevAgents& ag, const DevSlots& slots, size_t V,
+                                          unsigned& corr_nodes) {
+    if (!w.active) return;
+    int* path = ag.path + (size_t)w.agent * ag.path_cap;
+    if (w.first) {
+        const int lenS = (int)((side == 0 ? w.ent.x : w.gS) >> 1);
+        if (side == 0) {
+            w.idx = lenS - 1;
+        } else {
+            const int len = lenS + (int)(w.ent.x >> 1) + 1;  // PathFinder.h:280-290
+            w.idx = lenS;
+            ag.path_len[w.agent] = len;
+            corr_nodes += (unsigned)len;
+        }
+        w.first = false;
+    }
+    if (w.n != w.term) {
+        path[w.idx] = w.n;
+        w.idx += side == 0 ? -1 : 1;
+        w.n = (int)w.ent.y;
+    }
+    if (w.n == w.term) {
+        if (side == 1) path[w.idx] = w.term;
+        w.active = false;
+    } else {
+        w.ent = slots.vis[(size_t)w.slot * V + w.n];
+    }
+}
+
+// At least 4 waves per SIMD (at most 128 VGPRs): 16,384 searches at T = 16
+// need 4 resident waves per SIMD to run one search per segment.
 template <int T>
-__global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlots slots, DevFrame fr,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_search(DevMap map, DevAgents ag, DevSlots slots, DevFrame fr,
                                                SearchParams sp, int persistent) {
     extern __shared__ unsigned long long q[];  // 64 * Q keys
     constexpr int RPW = kWave / T;
-    constexpr int CS = cand_stride<T>();
     constexpr int HB = hash_bits<T>();
     constexpr int HS = 1 << HB;
-    __shared__ __align__(16) int cand[RPW * CS];
     __shared__ unsigned long long stage[kWave * 3];
-    __shared__ int htab[RPW * HS];
-    for (int i = threadIdx.x; i < RPW * HS; i += kWave) htab[i] = kHashEmpty;
+    __shared__ int hkey[RPW * HS];  // node of the slot, kHashEmpty when free
+    __shared__ int hval[RPW * HS];  // min over the node's candidates of (k << 1 | dir)
+    // per-lane target of the table operations of an empty neighbour slot, so
+    // the three slots' operations issue together without branches; it never
+    // holds kHashEmpty, so a compare-and-swap on it never writes
+    __shared__ int hdummy[kWave];
+    for (int i = threadIdx.x; i < RPW * HS; i += kWave) {
+        hkey[i] = kHashEmpty;
+        hval[i] = kHashNone;
+    }
+    hdummy[threadIdx.x] = kHashNone;
 
     const int L = threadIdx.x;
     const int sub = L / T;
     const int l = L % T;
     const int segbase = sub * 3 * T;
-    const int cbase = sub * CS;
     const unsigned long long smask = seg_mask<T>(sub);
     const unsigned long long below = smask & ((1ull << L) - 1ull);
     const int Q = sp.Q;
@@ -198,13 +251,17 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
     const int n_active = fr.counters[1];
 
     // per-segment search state (replicated in every lane of the segment)
-    int slot = -1;
+    int slot = -1, agent = -1;
     bool has_req = false, no_more = false, took_plain = false;
+    // the node this lane expands in the coming round: popped at the end of
+    // the previous round (the heap is not touched in between), or at the top
+    // of the first round after a search is taken up
+    bool need_pop = false, popped_next = false;
+    int c_next = -1;
     int S = 0, Tg = 0, cnt = 0;
     unsigned mval = 0;  // this search's mark generation, 1..127
     float2 cS = make_float2(0.f, 0.f), cT = make_float2(0.f, 0.f);
-    int rounds_here = 0;
-    long long rounds_tot = 0, exp_here = 0, disc_here = 0;
+    int rounds_here = 0, exp_here = 0, disc_here = 0;
     int status = 0;
     uint2* vis = slots.vis;
     unsigned char* mark = slots.mark;
@@ -215,6 +272,10 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
     int2 pf_r2 = make_int2(0, 0);
     uint2 pf_vc = make_uint2(0u, 0u);
     unsigned pf_vn[3] = {0u, 0u, 0u};
+    // corridor walk of this lane's side of the segment's last found search
+    Walk walk;
+    // frame statistics, added to fr.stats once per workgroup at the end
+    unsigned st_found = 0, st_fail = 0, st_exp = 0, st_disc = 0, st_corr = 0, st_rounds = 0;
 
     while (true) {
         // ---- acquire work for segments that have none
@@ -231,11 +292,11 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 no_more = true;
             } else {
                 slot = fr.active[item];
+                agent = slots.agent[slot];
                 S = slots.S[slot];
                 Tg = slots.Tg[slot];
                 mval = slots.stamp[slot] % 127u + 1u;
                 status = slots.status[slot];
-                rounds_tot = slots.rounds[slot];
                 cS = map.cen[S];
                 cT = map.cen[Tg];
                 rounds_here = 0;
@@ -269,19 +330,25 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                     for (int i = 0; i < cnt; ++i) q[i * kWave + L] = src[i];
                 }
                 has_req = true;
+                need_pop = true;
             }
         }
         if (__ballot(has_req) == 0ull) break;
         wave_sync();  // seeds, restored queues and visited writes visible
 
         // ---- pop + neighbour fetch
-        const bool popped = has_req && cnt > 0;
+        if (has_req && need_pop) {
+            popped_next = cnt > 0;
+            if (popped_next) c_next = key_node(heap_pop(q, L, cnt));
+            need_pop = false;
+        }
+        const bool popped = has_req && popped_next;
         int c = -1, G = 0, cdir = 0;
         int nn[3] = {-1, -1, -1};
         float2 cnb[3];
         bool have_vn = false;
         if (popped) {
-            c = key_node(heap_pop(q, L, cnt));
+            c = c_next;
             int4 r0 = pf_r0, r1 = pf_r1;
             int2 r2 = pf_r2;
             uint2 vc = pf_vc;
@@ -301,6 +368,13 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
             cnb[1] = make_float2(__int_as_float(r1.z), __int_as_float(r1.w));
             cnb[2] = make_float2(__int_as_float(r2.x), __int_as_float(r2.y));
         }
+        // marks of the neighbours: prefetched, or read now (all three loads
+        // issued before the first is used)
+        unsigned mk[3] = {pf_vn[0], pf_vn[1], pf_vn[2]};
+        if (popped && !have_vn) {
+#pragma unroll
+            for (int s = 0; s < 3; ++s) mk[s] = mark[nn[s] >= 0 ? nn[s] : 0];
+        }
         float Fn[3];
         bool prior[3];
         int pdir[3];
@@ -311,62 +385,76 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
             pdir[s] = 0;
             if (nn[s] >= 0) {
                 const float2 cn = cnb[s];
-                const unsigned mk = have_vn ? pf_vn[s] : (unsigned)mark[nn[s]];
                 const float h = cdir ? dist2(cn, cT) : dist2(cn, cS);
                 Fn[s] = (float)(G + 1) + h * sp.K;
-                prior[s] = (mk >> 1) == mval;
-                pdir[s] = (int)(mk & 1u);
+                prior[s] = (mk[s] >> 1) == mval;
+                pdir[s] = (int)(mk[s] & 1u);
             }
         }
-        int* hseg = &htab[sub * HS];
-        unsigned hslot[3] = {0u, 0u, 0u};
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            cand[cbase + l * 3 + s] = nn[s] >= 0 ? (nn[s] * 2 + cdir) : -1;
-            if (nn[s] >= 0 && !prior[s]) {  // only unvisited nodes can be claimed
-                hslot[s] = claim_hash(nn[s], HB);
-                atomicMin(&hseg[hslot[s]], (nn[s] << 8) | (l * 3 + s));
-            }
-        }
-        if (l == T - 1 && CS > 3 * T) {
-            for (int k = 3 * T; k < CS; ++k) cand[cbase + k] = -1;
-        }
-        wave_sync();
 
-        // ---- duplicate claims of this round: first earlier candidate with the
-        // same node, from the hash; a slot taken by another node falls back to
-        // scanning the earlier candidates
-        int first[3] = {-1, -1, -1};
+        // ---- this round's candidates on unmarked nodes: place each node in
+        // the segment's table (linear probing, first compare-and-swap wins
+        // the slot), then keep the smallest (lane, slot) index per node
+        int* hk = &hkey[sub * HS];
+        int* hv = &hval[sub * HS];
+        unsigned hslot[3];
+        bool ins[3], pend[3];
+        int* kp[3];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-            if (nn[s] < 0 || prior[s]) continue;
-            const int e = hseg[hslot[s]];
-            const int k = l * 3 + s;
-            if ((e >> 8) == nn[s]) {
-                const int fk = e & 0xff;
-                if (fk < k) first[s] = cand[cbase + fk];
-            } else {
-                for (int kk = 0; kk < k; ++kk) {
-                    const int cv = cand[cbase + kk];
-                    if (cv >= 0 && (cv >> 1) == nn[s]) {
-                        first[s] = cv;
-                        break;
-                    }
+            ins[s] = nn[s] >= 0 && !prior[s];
+            hslot[s] = claim_hash(nn[s] >= 0 ? nn[s] : 0, HB);
+            kp[s] = ins[s] ? &hk[hslot[s]] : &hdummy[L];
+        }
+        {
+            int old[3];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) old[s] = atomicCAS(kp[s], kHashEmpty, nn[s]);
+#pragma unroll
+            for (int s = 0; s < 3; ++s) pend[s] = ins[s] && old[s] != kHashEmpty && old[s] != nn[s];
+        }
+        // slot taken by another node: probe on (rare)
+        while (pend[0] || pend[1] || pend[2]) {
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                if (pend[s]) {
+                    hslot[s] = (hslot[s] + 1u) & (unsigned)(HS - 1);
+                    const int old = atomicCAS(&hk[hslot[s]], kHashEmpty, nn[s]);
+                    pend[s] = old != kHashEmpty && old != nn[s];
                 }
             }
         }
+        int* vp[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            vp[s] = ins[s] ? &hv[hslot[s]] : &hdummy[L];
+            atomicMin(vp[s], ((l * 3 + s) << 1) | cdir);
+        }
+        wave_sync();
+        int ev[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) ev[s] = *vp[s];
+
+        // ---- claims and meets.  A candidate claims an unmarked node when it
+        // is the node's first candidate in (lane, slot) order; otherwise the
+        // node's owner is the earlier marker (prior) or the first candidate,
+        // and a different direction is a meet.
         bool claim[3], meet[3];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
             claim[s] = false;
             meet[s] = false;
             if (nn[s] < 0) continue;
-            if (!prior[s] && first[s] < 0) {
-                claim[s] = true;
-            } else {
-                const int odir = prior[s] ? pdir[s] : (first[s] & 1);
-                meet[s] = (odir != cdir);
+            int odir = pdir[s];
+            if (ins[s]) {
+                const int e = ev[s];
+                if ((e >> 1) == l * 3 + s) {
+                    claim[s] = true;
+                    continue;
+                }
+                odir = e & 1;
             }
+            meet[s] = (odir != cdir);
         }
 
         // ---- claims: frontier order = (lane, slot) order
@@ -387,12 +475,14 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 }
             }
         }
-        // ---- first meet in (lane, slot) order
+        // ---- first meet in (lane, slot) order; the lanes exchange it only in
+        // rounds where some search of the workgroup meets
         const int ms = meet[0] ? 0 : (meet[1] ? 1 : (meet[2] ? 2 : 3));
-        const unsigned long long mb = __ballot(ms < 3) & smask;
+        const unsigned long long mball = __ballot(ms < 3);
+        const unsigned long long mb = mball & smask;
         const bool found = has_req && (mb != 0ull);
         int mS = -1, mT = -1;
-        {
+        if (mball != 0ull) {
             const int w = found ? __ffsll((long long)mb) - 1 : sub * T;
             const int n = ms < 3 ? nn[ms] : -1;
             const int a = cdir ? c : n, b = cdir ? n : c;
@@ -401,10 +491,14 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
         }
         const unsigned long long pb = __ballot(popped) & smask;
         wave_sync();
-        // empty this round's claim-hash slots for the next round
+        // empty this round's table slots for the next round
 #pragma unroll
-        for (int s = 0; s < 3; ++s)
-            if (nn[s] >= 0 && !prior[s]) hseg[hslot[s]] = kHashEmpty;
+        for (int s = 0; s < 3; ++s) {
+            if (ins[s]) {
+                hk[hslot[s]] = kHashEmpty;
+                hv[hslot[s]] = kHashNone;
+            }
+        }
 
         // ---- deal frontier item f to lane f mod T.  The heap top after the
         // pushes is min(current top, dealt items) (an item below the top is
@@ -436,10 +530,12 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 pf_vn_ok = true;
             }
         }
+        // one parent step of a pending corridor walk; its load was issued a
+        // round ago
+        if (l < 2) walk_step(walk, l, ag, slots, V, st_corr);
         const unsigned long long nb = __ballot(cnt > 0) & smask;
         if (has_req) {
             ++rounds_here;
-            ++rounds_tot;
             exp_here += __popcll(pb);
             disc_here += tot;
             if (found) {
@@ -456,54 +552,65 @@ __global__ __launch_bounds__(64) void k_search(DevMap map, DevAgents ag, DevSlot
                 for (int i = 0; i < cnt; ++i) dst[i] = q[i * kWave + L];
             }
             if (ended && status == 1 && l < 2) {
-                // PathFinder.h:276-306: lane 0 writes the start side, lane 1 the target side
-                const int a = slots.agent[slot];
-                int* path = ag.path + (size_t)a * ag.path_cap;
-                const int lenS = (int)(vis[mS].x >> 1);
-                if (l == 0) {
-                    int idx = lenS - 1, n = mS;
-                    while (n != S) {
-                        path[idx--] = n;
-                        n = (int)vis[n].y;
-                    }
-                } else {
-                    int idx = lenS, n = mT;
-                    while (n != Tg) {
-                        path[idx++] = n;
-                        n = (int)vis[n].y;
-                    }
-                    path[idx] = Tg;
-                }
+                // a walk still pending on these lanes (their previous search
+                // in a persistent segment) finishes first
+                while (walk.active) walk_step(walk, l, ag, slots, V, st_corr);
+                walk.active = true;
+                walk.first = true;
+                walk.agent = agent;
+                walk.slot = slot;
+                walk.n = l == 0 ? mS : mT;
+                walk.term = l == 0 ? S : Tg;
+                walk.ent = vis[walk.n];
+                if (l == 1) walk.gS = vis[mS].x;
             }
             if (l == 0 && (ended || park)) {
                 slots.status[slot] = status;
-                slots.rounds[slot] = rounds_tot;
-                slots.expansions[slot] += exp_here;
-                slots.discoveries[slot] += disc_here;
-                atomicAdd(&fr.stats[2], (unsigned long long)exp_here);
-                atomicAdd(&fr.stats[3], (unsigned long long)disc_here);
-                atomicAdd(&fr.stats[5], (unsigned long long)rounds_here);
+                st_exp += (unsigned)exp_here;
+                st_disc += (unsigned)disc_here;
+                st_rounds += (unsigned)rounds_here;
                 if (ended) {
-                    const int a = slots.agent[slot];
-                    if (status == 1) {  // PathFinder.h:280-290
-                        const int len = (int)(vis[mS].x >> 1) + (int)(vis[mT].x >> 1) + 1;
-                        slots.mS[slot] = mS;
-                        slots.mT[slot] = mT;
-                        ag.path_len[a] = len;
-                        ag.state[a] = kMove;
-                        ag.step_idx[a] = -1;
-                        atomicAdd(&fr.stats[0], 1ull);
-                        atomicAdd(&fr.stats[4], (unsigned long long)len);
+                    if (status == 1) {  // PathFinder.h:280-290 (the length is written by the walk)
+                        ag.state[agent] = kMove;
+                        ag.step_idx[agent] = -1;
+                        ++st_found;
                     } else {  // PathFinder.h:268-274
-                        ag.state[a] = kIdle;
-                        atomicAdd(&fr.stats[1], 1ull);
+                        ag.state[agent] = kIdle;
+                        ++st_fail;
                     }
                     slots.agent[slot] = -1;
-                    ag.inflight[a] = -1;
+                    ag.inflight[agent] = -1;
                 }
             }
             if (ended || park) has_req = false;
         }
+        // ---- next round's pop
+        popped_next = false;
+        if (has_req && cnt > 0) {
+            c_next = key_node(heap_pop(q, L, cnt));
+            popped_next = true;
+        }
+    }
+    // walks still pending when the last search of the workgroup ended: the
+    // lanes step them together
+    if (l < 2)
+        while (walk.active) walk_step(walk, l, ag, slots, V, st_corr);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        st_found += __shfl_xor(st_found, o, kWave);
+        st_fail += __shfl_xor(st_fail, o, kWave);
+        st_exp += __shfl_xor(st_exp, o, kWave);
+        st_disc += __shfl_xor(st_disc, o, kWave);
+        st_corr += __shfl_xor(st_corr, o, kWave);
+        st_rounds += __shfl_xor(st_rounds, o, kWave);
+    }
+    if (L == 0) {
+        if (st_found) atomicAdd(&fr.stats[0], (unsigned long long)st_found);
+        if (st_fail) atomicAdd(&fr.stats[1], (unsigned long long)st_fail);
+        if (st_exp) atomicAdd(&fr.stats[2], (unsigned long long)st_exp);
+        if (st_disc) atomicAdd(&fr.stats[3], (unsigned long long)st_disc);
+        if (st_corr) atomicAdd(&fr.stats[4], (unsigned long long)st_corr);
+        if (st_rounds) atomicAdd(&fr.stats[5], (unsigned long long)st_rounds);
     }
 }
 
