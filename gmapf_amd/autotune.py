@@ -42,7 +42,8 @@ BUDGETS = (48, 64, 96, 128, 192, 256)
 
 
 def lds_fits(Q):
-    return 64 * Q * 8 + 4096 <= LDS_BYTES
+    # Q heap rows + one spare row of 64 keys, plus about 10 KB of static tables
+    return 64 * (Q + 1) * 8 + 10 * 1024 <= LDS_BYTES
 
 
 def _confirm(sc, rows, warmup, steps, log):

@@ -65,71 +65,77 @@ __device__ __forceinline__ float dist2(float2 a, float2 b) {
     return x * x + y * y;
 }
 
-// ---- per-lane binary min-heap in LDS, "keep the Q smallest" (CuQueue.h:33-70)
-__device__ __forceinline__ void heap_sift_up(unsigned long long* q, int L, int i, unsigned long long k) {
-    while (i > 0) {
+// ---- per-lane binary min-heap in LDS, "keep the Q smallest" (CuQueue.h:33-70).
+// Entry i of lane L is q[i * 64 + L]; row Q is the lane's spare entry, the
+// target of the writes of a lane that has nothing to move, so the sifts run
+// a fixed number of levels without branches and the wave leaves a sift as
+// soon as no lane moves (a ballot per level).  All lanes of the wave call
+// these functions; `on` selects the lanes that act.
+constexpr unsigned long long kKeyNone = ~0ull;  // larger than every key
+
+// Sift key k up from hole i.
+__device__ __forceinline__ void heap_sift_up(unsigned long long* q, int L, int Q, int depth, bool on, int i,
+                                             unsigned long long k) {
+    bool done = !on;
+    for (int d = 0; d < depth; ++d) {
+        if (__ballot(!done) == 0ull) break;
+        const bool hp = i > 0;
         const int p = (i - 1) >> 1;
-        const unsigned long long v = q[p * kWave + L];
-        if (v < k) break;
-        q[i * kWave + L] = v;
-        i = p;
+        const unsigned long long pv = q[(hp ? p : Q) * kWave + L];
+        const bool move = !done && hp && pv > k;
+        q[(move ? i : Q) * kWave + L] = pv;
+        i = move ? p : i;
+        done = !move;
     }
-    q[i * kWave + L] = k;
+    q[(on ? i : Q) * kWave + L] = k;
 }
 
-__device__ __forceinline__ void heap_push(unsigned long long* q, int L, int& cnt, int Q, unsigned long long k) {
-    if (cnt < Q) {
-        heap_sift_up(q, L, cnt, k);
-        ++cnt;
-        return;
-    }
-    // full: the maximum is a leaf (indices Q/2 .. Q-1); read 4 leaves per step
-    int mi = Q >> 1;
-    unsigned long long mv = q[mi * kWave + L];
-    int i = mi + 1;
-    for (; i + 4 <= Q; i += 4) {
-        const unsigned long long v0 = q[i * kWave + L], v1 = q[(i + 1) * kWave + L];
-        const unsigned long long v2 = q[(i + 2) * kWave + L], v3 = q[(i + 3) * kWave + L];
-        if (v0 > mv) { mv = v0; mi = i; }
-        if (v1 > mv) { mv = v1; mi = i + 1; }
-        if (v2 > mv) { mv = v2; mi = i + 2; }
-        if (v3 > mv) { mv = v3; mi = i + 3; }
-    }
-    for (; i < Q; ++i) {
-        const unsigned long long v = q[i * kWave + L];
-        if (v > mv) {
-            mv = v;
-            mi = i;
-        }
-    }
-    if (k > mv) return;  // not smaller than the maximum: dropped
-    heap_sift_up(q, L, mi, k);
-}
-
-__device__ __forceinline__ unsigned long long heap_pop(unsigned long long* q, int L, int& cnt) {
-    const unsigned long long top = q[L];
-    --cnt;
-    const unsigned long long last = q[cnt * kWave + L];
+// Sift key k down from a hole at the root of a heap of n items.
+__device__ __forceinline__ void heap_sift_down(unsigned long long* q, int L, int Q, int depth, bool on, int n,
+                                               unsigned long long k) {
     int i = 0;
-    while (true) {
-        const int l = 2 * i + 1;
-        if (l >= cnt) break;
-        const int r = l + 1;
-        unsigned long long cv = q[l * kWave + L];
-        int c = l;
-        if (r < cnt) {
-            const unsigned long long rv = q[r * kWave + L];
-            if (rv < cv) {
-                cv = rv;
-                c = r;
-            }
-        }
-        if (!(cv < last)) break;
-        q[i * kWave + L] = cv;
-        i = c;
+    bool done = !on;
+    for (int d = 0; d < depth; ++d) {
+        if (__ballot(!done) == 0ull) break;
+        const int l = 2 * i + 1, r = l + 1;
+        unsigned long long lv = q[(l < Q ? l : Q) * kWave + L];
+        unsigned long long rv = q[(r < Q ? r : Q) * kWave + L];
+        lv = l < n ? lv : kKeyNone;
+        rv = r < n ? rv : kKeyNone;
+        const bool ur = rv < lv;
+        const unsigned long long cv = ur ? rv : lv;
+        const bool move = !done && cv < k;
+        q[(move ? i : Q) * kWave + L] = cv;
+        i = move ? (ur ? r : l) : i;
+        done = !move;
     }
-    if (cnt > 0) q[i * kWave + L] = last;
-    return top;
+    q[(on ? i : Q) * kWave + L] = k;
+}
+
+// Push key k (lanes with k != kKeyNone) into a heap bounded to `cap` items:
+// below the bound it is added; at the bound it replaces the largest item (a
+// leaf, indices cap/2 .. cap-1) when smaller, else it is dropped.
+__device__ __forceinline__ void heap_push(unsigned long long* q, int L, int Q, int depth, int cap, int& cnt,
+                                          unsigned long long k) {
+    const bool on = k != kKeyNone;
+    const bool full = cnt >= cap;
+    bool go = on;
+    int start = cnt;
+    if (__ballot(on && full) != 0ull) {
+        int mi = 0;
+        unsigned long long mv = 0ull;  // with no leaf (cap 0) every key is dropped
+        for (int j = cap >> 1; j < cap; ++j) {
+            const unsigned long long v = q[j * kWave + L];
+            mi = v > mv ? j : mi;
+            mv = v > mv ? v : mv;
+        }
+        if (full) {
+            go = on && k < mv;
+            start = mi;
+        }
+    }
+    heap_sift_up(q, L, Q, depth, go, start, k);
+    cnt += (on && !full) ? 1 : 0;
 }
 
 // Ordering point between the lanes of the workgroup.  A workgroup is one
@@ -154,12 +160,12 @@ __device__ __forceinline__ unsigned long long seg_mask(int sub) {
     }
 }
 
-// per-segment claim table: open addressing, power of two >= 16T slots for the
-// at most 3T candidates of a round (load factor <= 3/16)
+// per-segment claim table: open addressing, 8T slots for the at most 3T
+// candidates of a round (load factor <= 3/8; 4 KB per workgroup for any T)
 template <int T>
 constexpr int hash_bits() {
     int b = 1;
-    while ((1 << b) < 16 * T) ++b;
+    while ((1 << b) < 8 * T) ++b;
     return b;
 }
 constexpr int kHashEmpty = -1;
@@ -223,7 +229,7 @@ __device__ __forceinline__ void walk_step(Walk& w, int side, const DevAgents& ag
 template <int T>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_search(DevMap map, DevAgents ag, DevSlots slots, DevFrame fr,
                                                SearchParams sp, int persistent) {
-    extern __shared__ unsigned long long q[];  // 64 * Q keys
+    extern __shared__ unsigned long long q[];  // 64 * (Q + 1) keys: Q heap rows + the spare row
     constexpr int RPW = kWave / T;
     constexpr int HB = hash_bits<T>();
     constexpr int HS = 1 << HB;
@@ -247,15 +253,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const unsigned long long smask = seg_mask<T>(sub);
     const unsigned long long below = smask & ((1ull << L) - 1ull);
     const int Q = sp.Q;
+    const int depth = 31 - __clz(Q);  // levels below the root of a Q-item heap
     const size_t V = (size_t)map.V;
     const int n_active = fr.counters[1];
 
     // per-segment search state (replicated in every lane of the segment)
     int slot = -1, agent = -1;
     bool has_req = false, no_more = false, took_plain = false;
-    // the node this lane expands in the coming round: popped at the end of
-    // the previous round (the heap is not touched in between), or at the top
-    // of the first round after a search is taken up
+    // the node this lane expands in the coming round: taken from the heap
+    // and the items dealt to the lane at the end of the previous round, or
+    // popped at the top of the first round of a fresh search
     bool need_pop = false, popped_next = false;
     int c_next = -1;
     int S = 0, Tg = 0, cnt = 0;
@@ -306,6 +313,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 mark = slots.mark + (size_t)slot * slots.VP;
                 pf_c = -1;
                 pf_vn_ok = false;
+                need_pop = status == 3;
+                popped_next = false;
                 if (status == 3) {
                     // PathFinder.h:168-189
                     const float F0 = dist2(cS, cT) * sp.K;
@@ -324,22 +333,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     }
                     status = 0;
                 } else {
+                    // parked: [node to expand next or kKeyNone, heap items]
                     const size_t qb = ((size_t)slot * T + l);
                     cnt = slots.qcnt[qb];
                     const unsigned long long* src = slots.qsave + qb * (size_t)Q;
-                    for (int i = 0; i < cnt; ++i) q[i * kWave + L] = src[i];
+                    const unsigned long long k0 = src[0];
+                    popped_next = k0 != kKeyNone;
+                    c_next = key_node(k0);
+                    for (int i = 0; i < cnt; ++i) q[i * kWave + L] = src[1 + i];
                 }
                 has_req = true;
-                need_pop = true;
             }
         }
         if (__ballot(has_req) == 0ull) break;
         wave_sync();  // seeds, restored queues and visited writes visible
 
-        // ---- pop + neighbour fetch
-        if (has_req && need_pop) {
-            popped_next = cnt > 0;
-            if (popped_next) c_next = key_node(heap_pop(q, L, cnt));
+        // ---- pop (first round of a fresh search) + neighbour fetch
+        if (__ballot(has_req && need_pop) != 0ull) {
+            const bool on = has_req && need_pop && cnt > 0;
+            if (on) {
+                c_next = key_node(q[L]);
+                --cnt;
+            }
+            const unsigned long long last = q[(on ? cnt : Q) * kWave + L];
+            heap_sift_down(q, L, Q, depth, on && cnt > 0, cnt, last);
+            if (has_req && need_pop) popped_next = on;
             need_pop = false;
         }
         const bool popped = has_req && popped_next;
@@ -500,40 +518,68 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
         }
 
-        // ---- deal frontier item f to lane f mod T.  The heap top after the
-        // pushes is min(current top, dealt items) (an item below the top is
-        // never dropped), so next round's node data is requested first and
-        // arrives while the pushes run.
-        if (has_req) {
-            unsigned long long nt = cnt > 0 ? q[L] : ~0ull;
-            for (int f = l; f < tot; f += T) {
-                const unsigned long long k = stage[segbase + f];
-                nt = k < nt ? k : nt;
-            }
-            pf_c = nt != ~0ull ? key_node(nt) : -1;
-            pf_vn_ok = false;
-            if (pf_c >= 0) {
-                pf_r0 = map.rec[3 * (size_t)pf_c];
-                pf_r1 = map.rec[3 * (size_t)pf_c + 1];
-                pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)pf_c + 2);
-                pf_vc = vis[pf_c];
-            }
-            for (int f = l; f < tot; f += T) heap_push(q, L, cnt, Q, stage[segbase + f]);
-            // visited entries of the predicted node's neighbours: every write
-            // of this round is visible (barrier above) and none can happen
-            // before the next round reads them
-            if (pf_c >= 0) {
-                const int a0 = pf_r0.x, a1 = a0 >= 0 ? pf_r0.y : -1, a2 = a1 >= 0 ? pf_r0.z : -1;
-                if (a0 >= 0) pf_vn[0] = mark[a0];
-                if (a1 >= 0) pf_vn[1] = mark[a1];
-                if (a2 >= 0) pf_vn[2] = mark[a2];
-                pf_vn_ok = true;
-            }
+        // ---- deal frontier item f to lane f mod T (at most three per lane),
+        // push, then pop: the round pushes its items into a heap bounded to Q
+        // and the next round pops the minimum.  That minimum m is min(heap
+        // top, dealt items), known before any push, so next round's node data
+        // is requested first and arrives while the heap is updated.  After
+        // the pop the heap holds ranks 2 .. min(n + k, Q) of the heap's n and
+        // the k dealt items; the same set results from keeping m out of the
+        // heap and pushing the other items with bound Q - 1 (n <= Q - 1 here,
+        // since every round ends with a pop).  When m is the top, the first
+        // dealt item takes its place (one sift down instead of a push and a
+        // pop).
+        const bool cont = has_req && !found;
+        unsigned long long d0 = stage[segbase + l], d1 = stage[segbase + l + T], d2 = stage[segbase + l + 2 * T];
+        d0 = cont && l < tot ? d0 : kKeyNone;
+        d1 = cont && l + T < tot ? d1 : kKeyNone;
+        d2 = cont && l + 2 * T < tot ? d2 : kKeyNone;
+        const unsigned long long top = cont && cnt > 0 ? q[L] : kKeyNone;
+        unsigned long long m = top < d0 ? top : d0;
+        m = d1 < m ? d1 : m;
+        m = d2 < m ? d2 : m;
+        const bool has_next = m != kKeyNone;
+        pf_c = has_next ? key_node(m) : -1;
+        pf_vn_ok = false;
+        if (has_next) {
+            pf_r0 = map.rec[3 * (size_t)pf_c];
+            pf_r1 = map.rec[3 * (size_t)pf_c + 1];
+            pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)pf_c + 2);
+            pf_vc = vis[pf_c];
+        }
+        {
+            const bool from_top = has_next && m == top;
+            // the root's replacement: the first dealt item, or the last item
+            const bool repl = from_top && d0 != kKeyNone;
+            const bool shrink = from_top && !repl;
+            if (shrink) --cnt;
+            const unsigned long long last = q[(shrink ? cnt : Q) * kWave + L];
+            heap_sift_down(q, L, Q, depth, repl || (shrink && cnt > 0), cnt, repl ? d0 : last);
+            // the items still to push: the dealt items other than m (and d0
+            // when it replaced the root)
+            const unsigned long long p0 = (repl || m == d0) ? kKeyNone : d0;
+            const unsigned long long p1 = m == d1 ? kKeyNone : d1;
+            const unsigned long long p2 = m == d2 ? kKeyNone : d2;
+            heap_push(q, L, Q, depth, Q - 1, cnt, p0);
+            heap_push(q, L, Q, depth, Q - 1, cnt, p1);
+            heap_push(q, L, Q, depth, Q - 1, cnt, p2);
+        }
+        popped_next = has_next;
+        c_next = pf_c;
+        // visited entries of the next node's neighbours: every write of this
+        // round is visible (barrier above) and none can happen before the
+        // next round reads them
+        if (has_next) {
+            const int a0 = pf_r0.x, a1 = a0 >= 0 ? pf_r0.y : -1, a2 = a1 >= 0 ? pf_r0.z : -1;
+            pf_vn[0] = mark[a0 >= 0 ? a0 : 0];
+            pf_vn[1] = mark[a1 >= 0 ? a1 : 0];
+            pf_vn[2] = mark[a2 >= 0 ? a2 : 0];
+            pf_vn_ok = true;
         }
         // one parent step of a pending corridor walk; its load was issued a
         // round ago
         if (l < 2) walk_step(walk, l, ag, slots, V, st_corr);
-        const unsigned long long nb = __ballot(cnt > 0) & smask;
+        const unsigned long long nb = __ballot(has_next) & smask;
         if (has_req) {
             ++rounds_here;
             exp_here += __popcll(pb);
@@ -546,10 +592,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const bool ended = status != 0;
             const bool park = !ended && sp.budget > 0 && rounds_here >= sp.budget;
             if (park) {
+                // [node to expand next or kKeyNone, heap items]: at most Q keys
                 const size_t qb = ((size_t)slot * T + l);
                 slots.qcnt[qb] = cnt;
                 unsigned long long* dst = slots.qsave + qb * (size_t)Q;
-                for (int i = 0; i < cnt; ++i) dst[i] = q[i * kWave + L];
+                dst[0] = m;
+                for (int i = 0; i < cnt; ++i) dst[1 + i] = q[i * kWave + L];
             }
             if (ended && status == 1 && l < 2) {
                 // a walk still pending on these lanes (their previous search
@@ -583,12 +631,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
             }
             if (ended || park) has_req = false;
-        }
-        // ---- next round's pop
-        popped_next = false;
-        if (has_req && cnt > 0) {
-            c_next = key_node(heap_pop(q, L, cnt));
-            popped_next = true;
         }
     }
     // walks still pending when the last search of the workgroup ended: the
@@ -628,7 +670,7 @@ const void* search_fn(int T) {
 
 }  // namespace
 
-int search_lds_bytes(int Q) { return kWave * Q * (int)sizeof(unsigned long long); }
+int search_lds_bytes(int Q) { return kWave * (Q + 1) * (int)sizeof(unsigned long long); }
 
 hipError_t search_prepare(int T, int Q) {
     const void* fn = search_fn(T);
