@@ -112,6 +112,10 @@ _SIGS = {
     "gmapf_fill_map_obstacle": [C.c_void_p, C.c_void_p, C.c_int],
     "gmapf_fill_map_end": [C.c_void_p],
     "gmapf_fill_map_manually": [C.c_void_p, C.c_void_p, C.c_int],
+    "gmapf_nav_encode": [C.c_void_p, C.c_void_p, C.c_longlong, C.POINTER(C.c_longlong)],
+    "gmapf_fill_map_from_nav": [C.c_void_p, C.c_void_p, C.c_longlong],
+    "gmapf_save_nav": [C.c_void_p, C.c_char_p],
+    "gmapf_load_nav": [C.c_void_p, C.c_char_p],
     "gmapf_add_agent": [C.c_void_p, C.c_float, C.c_float, C.POINTER(C.c_int)],
     "gmapf_set_agent_targ_pos": [C.c_void_p, C.c_int, C.c_float, C.c_float],
     "gmapf_async_step": [C.c_void_p, C.c_float],
@@ -242,6 +246,24 @@ class GMAPF:
         buf["p"][:, :6] = tri
         buf["n"][:] = nbr
         self._check(self._lib.gmapf_fill_map_manually(self._h, nodes, n))
+
+    # -- map fixture ("GMAPFNAV" byte string, no reference counterpart)
+    def nav_bytes(self) -> bytes:
+        n = C.c_longlong()
+        self._check(self._lib.gmapf_nav_encode(self._h, None, 0, C.byref(n)))
+        buf = (C.c_ubyte * n.value)()
+        self._check(self._lib.gmapf_nav_encode(self._h, buf, n.value, C.byref(n)))
+        return bytes(buf)
+
+    def fill_map_from_nav(self, data: bytes):
+        data = bytes(data)
+        self._check(self._lib.gmapf_fill_map_from_nav(self._h, data, len(data)))
+
+    def save_nav(self, path: str):
+        self._check(self._lib.gmapf_save_nav(self._h, os.fsencode(path)))
+
+    def load_nav(self, path: str):
+        self._check(self._lib.gmapf_load_nav(self._h, os.fsencode(path)))
 
     # -- agents
     def add_agent(self, x: float, y: float) -> int:

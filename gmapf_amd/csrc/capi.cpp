@@ -17,6 +17,7 @@
 #include "../../include/gmapf_c.h"
 #include "device.h"
 #include "kernels.h"
+#include "navfile.h"
 #include "navmesh.h"
 
 using namespace gmapf_amd;
@@ -914,6 +915,76 @@ int gmapf_get_agent_steps(gmapf_handle h, int* step_idx, int* step_node, float* 
         if (step_idx) HIPCHK(hipMemcpy(step_idx, h->dag.step_idx, N * sizeof(int), hipMemcpyDeviceToHost));
         if (step_node) HIPCHK(hipMemcpy(step_node, h->dag.step_node, N * sizeof(int), hipMemcpyDeviceToHost));
         if (step_xy) HIPCHK(hipMemcpy(step_xy, h->dag.step_pos, N * sizeof(float2), hipMemcpyDeviceToHost));
+    });
+}
+
+int gmapf_nav_encode(gmapf_handle h, void* buf, long long cap, long long* len) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!h->map_filled) throw ApiError(GMAPF_E_STATE, "map is not filled");
+        const std::vector<unsigned char> b =
+            nav_encode(h->nav.V, h->nav.tri.data(), h->nav.nbr.data(), h->nav.cen.data());
+        if (len) *len = (long long)b.size();
+        if (buf) {
+            if (cap < (long long)b.size()) throw ApiError(GMAPF_E_ARG, "buffer too small for the navmesh");
+            std::memcpy(buf, b.data(), b.size());
+        }
+    });
+}
+
+int gmapf_fill_map_from_nav(gmapf_handle h, const void* buf, long long len) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!buf || len <= 0) throw ApiError(GMAPF_E_ARG, "no navmesh bytes");
+        int V = 0;
+        std::vector<float> tri6;
+        std::vector<int> nbr3;
+        try {
+            nav_decode(static_cast<const unsigned char*>(buf), (size_t)len, V, tri6, nbr3);
+        } catch (const std::runtime_error& e) {
+            throw ApiError(GMAPF_E_ARG, e.what());
+        }
+        h->fill_nodes(V, tri6.data(), nbr3.data());
+    });
+}
+
+int gmapf_save_nav(gmapf_handle h, const char* path) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!path) throw ApiError(GMAPF_E_ARG, "null path");
+        if (!h->map_filled) throw ApiError(GMAPF_E_STATE, "map is not filled");
+        const std::vector<unsigned char> b =
+            nav_encode(h->nav.V, h->nav.tri.data(), h->nav.nbr.data(), h->nav.cen.data());
+        FILE* f = std::fopen(path, "wb");
+        if (!f) throw ApiError(GMAPF_E_ARG, std::string("cannot open ") + path + " for writing");
+        const size_t w = std::fwrite(b.data(), 1, b.size(), f);
+        const int rc = std::fclose(f);
+        if (w != b.size() || rc != 0) throw ApiError(GMAPF_E_ARG, std::string("cannot write ") + path);
+    });
+}
+
+int gmapf_load_nav(gmapf_handle h, const char* path) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!path) throw ApiError(GMAPF_E_ARG, "null path");
+        FILE* f = std::fopen(path, "rb");
+        if (!f) throw ApiError(GMAPF_E_ARG, std::string("cannot open ") + path);
+        std::vector<unsigned char> b;
+        unsigned char chunk[1 << 16];
+        size_t r;
+        while ((r = std::fread(chunk, 1, sizeof(chunk), f)) > 0) b.insert(b.end(), chunk, chunk + r);
+        const bool err = std::ferror(f) != 0;
+        std::fclose(f);
+        if (err) throw ApiError(GMAPF_E_ARG, std::string("cannot read ") + path);
+        int V = 0;
+        std::vector<float> tri6;
+        std::vector<int> nbr3;
+        try {
+            nav_decode(b.data(), b.size(), V, tri6, nbr3);
+        } catch (const std::runtime_error& e) {
+            throw ApiError(GMAPF_E_ARG, std::string(path) + ": " + e.what());
+        }
+        h->fill_nodes(V, tri6.data(), nbr3.data());
     });
 }
 

@@ -65,6 +65,9 @@ struct gmapf_server_s {
     long long n_submitted = 0, n_admitted = 0, n_completed = 0, n_found = 0, n_nopath = 0,
               n_rejected = 0, held_by_budget = 0;
     long long live_bytes = 0, peak_bytes = 0;
+    std::vector<long long> credit;  // fair policy: per-map byte credit
+    long long quantum = 0;          // fair policy: credit per admission (largest footprint)
+    long long rr = 0;               // fair policy: admissions so far (rotates the first map)
     bool adopted = false, going = false;
     Clock::time_point t_frame;
     double last_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -78,6 +81,8 @@ struct gmapf_server_s {
         if (!hs || n < 1) throw ServeError(GMAPF_E_ARG, "no maps");
         cfg = c ? *c : gmapf_server_config{};
         if (cfg.max_pending <= 0) cfg.max_pending = 1 << 22;
+        if (cfg.admission_policy != GMAPF_ADMIT_FIFO && cfg.admission_policy != GMAPF_ADMIT_FAIR)
+            throw ServeError(GMAPF_E_ARG, "unknown admission policy");
         for (int m = 0; m < n; m++) {
             gmapf_handle h = hs[m];
             if (!h) throw ServeError(GMAPF_E_ARG, "null map handle");
@@ -95,6 +100,8 @@ struct gmapf_server_s {
             slots.push_back(gc.path_finder_parallel_agents);
             live.push_back(0);
             busy.emplace_back();
+            credit.push_back(0);
+            quantum = std::max(quantum, foot.back());
         }
     }
 
@@ -146,8 +153,12 @@ struct gmapf_server_s {
     }
 
     // Admission scan (see gmapf_serve.h); `take` receives admitted positions
-    // in `pending`.
-    void plan(std::vector<size_t>& take, long long* held) {
+    // in `pending`.  `commit` false: a dry run that changes no state.
+    void plan(std::vector<size_t>& take, long long* held, bool commit) {
+        if (cfg.admission_policy == GMAPF_ADMIT_FAIR) {
+            plan_fair(take, held, commit);
+            return;
+        }
         std::vector<int> free_slots(maps.size());
         for (size_t m = 0; m < maps.size(); m++) free_slots[m] = slots[m] - live[m];
         long long bytes = live_bytes;
@@ -170,13 +181,77 @@ struct gmapf_server_s {
         if (held) *held = h;
     }
 
+    void plan_fair(std::vector<size_t>& take, long long* held, bool commit) {
+        const size_t M = maps.size();
+        const long long budget = cfg.queue_memory_budget;
+        std::vector<std::vector<size_t>> q(M);  // per map: positions in `pending`, arrival order
+        for (size_t i = 0; i < pending.size(); i++) q[pending[i].map].push_back(i);
+        std::vector<size_t> head(M, 0);
+        std::vector<int> free_slots(M);
+        std::vector<long long> d(credit);
+        for (size_t m = 0; m < M; m++) {
+            free_slots[m] = slots[m] - live[m];
+            d[m] = q[m].empty() ? 0 : std::min(d[m] + quantum, 2 * quantum);
+        }
+        long long bytes = live_bytes;
+        bool blocked = false;
+        // map m's next request not in flight and not taken by this scan
+        auto next = [&](size_t m) -> long long {
+            while (head[m] < q[m].size() && busy[m][pending[q[m][head[m]]].agent]) head[m]++;
+            return head[m] < q[m].size() ? (long long)q[m][head[m]] : -1;
+        };
+        auto admit = [&](size_t m, size_t i) {
+            take.push_back(i);
+            busy[m][pending[i].agent] = 2;
+            head[m]++;
+            bytes += foot[m];
+            free_slots[m]--;
+            d[m] -= foot[m];
+        };
+        const size_t start = M ? (size_t)(rr % (long long)M) : 0;
+        // fair passes: turns in rotation while credit covers the next request
+        for (bool progress = true; progress && !blocked;) {
+            progress = false;
+            for (size_t j = 0; j < M && !blocked; j++) {
+                const size_t m = (start + j) % M;
+                if (free_slots[m] <= 0 || (budget > 0 && d[m] < foot[m])) continue;
+                const long long i = next(m);
+                if (i < 0) continue;
+                if (budget > 0 && bytes + foot[m] > budget) {
+                    blocked = true;
+                    break;
+                }
+                admit(m, (size_t)i);
+                progress = true;
+            }
+        }
+        // memory left: the rest in arrival order, charged to the maps' credit
+        if (!blocked) {
+            for (size_t i = 0; i < pending.size(); i++) {
+                const size_t m = pending[i].map;
+                if (busy[m][pending[i].agent] || free_slots[m] <= 0) continue;
+                if (budget > 0 && bytes + foot[m] > budget) {
+                    blocked = true;
+                    break;
+                }
+                admit(m, i);
+            }
+        }
+        for (size_t k : take) busy[pending[k].map][pending[k].agent] = 0;
+        if (held) *held = blocked ? (long long)(pending.size() - take.size()) : 0;
+        if (commit) {
+            for (size_t m = 0; m < M; m++) credit[m] = std::max(d[m], -4 * quantum);
+            rr++;
+        }
+    }
+
     void async_step(float dt) {
         if (going) return;  // like GMAPF::AsyncStep during a step (GMAPF.cu:260-261)
         t_frame = Clock::now();
         for (size_t m = 0; m < maps.size(); m++) agents_of((int)m);
         adopt();
         std::vector<size_t> take;
-        plan(take, &held_by_budget);
+        plan(take, &held_by_budget, true);
         // per map: ids and targets of this frame's admissions
         std::vector<std::vector<int>> ids(maps.size());
         std::vector<std::vector<float>> xy(maps.size());
@@ -390,7 +465,7 @@ int gmapf_server_plan(gmapf_server s, long long* tickets, int cap, int* n) {
         for (size_t m = 0; m < s->maps.size(); m++) s->agents_of((int)m);
         s->adopt();
         std::vector<size_t> take;
-        s->plan(take, nullptr);
+        s->plan(take, nullptr, false);
         for (int k = 0; tickets && k < cap && k < (int)take.size(); k++) tickets[k] = s->pending[take[k]].ticket;
         if (n) *n = (int)take.size();
     });

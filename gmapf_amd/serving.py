@@ -14,12 +14,16 @@ import numpy as np
 from . import GMAPF, MODE_BATCHED, Config, GmapfError, load_library
 from . import scenarios as S
 
-__all__ = ["Server", "ServingMap", "serving_maps", "build_map", "run_serving", "DONE_DTYPE"]
+__all__ = ["Server", "ServingMap", "serving_maps", "build_map", "run_serving", "DONE_DTYPE", "ADMIT_FIFO",
+           "ADMIT_FAIR"]
 
 
 class _ServerConfig(C.Structure):
     _fields_ = [("queue_memory_budget", C.c_longlong), ("max_pending", C.c_int),
-                ("record_start_nodes", C.c_int)]
+                ("record_start_nodes", C.c_int), ("admission_policy", C.c_int)]
+
+
+ADMIT_FIFO, ADMIT_FAIR = 0, 1
 
 
 class _Done(C.Structure):
@@ -45,11 +49,13 @@ class Server:
     """gmapf_server over GMAPF engines (one per map); the server owns their
     frame loop (see include/gmapf_serve.h)."""
 
-    def __init__(self, maps, queue_memory_budget=0, max_pending=0, record_start_nodes=False):
+    def __init__(self, maps, queue_memory_budget=0, max_pending=0, record_start_nodes=False,
+                 admission_policy=ADMIT_FIFO):
         self._lib = load_library()
         self.maps = list(maps)
         hs = (C.c_void_p * len(self.maps))(*[m._h for m in self.maps])
-        cfg = _ServerConfig(int(queue_memory_budget), int(max_pending), int(bool(record_start_nodes)))
+        cfg = _ServerConfig(int(queue_memory_budget), int(max_pending), int(bool(record_start_nodes)),
+                            int(admission_policy))
         s = C.c_void_p()
         rc = self._lib.gmapf_server_create(C.byref(s), hs, len(self.maps), C.byref(cfg))
         if rc != 0:

@@ -114,6 +114,20 @@ class GMAPF {
     void FillMapManually(const std::vector<CuNodesMap::Node>& nodes) {
         check(gmapf_fill_map_manually(h_, reinterpret_cast<const gmapf_node*>(nodes.data()), (int)nodes.size()));
     }
+    // Map fixture (no reference counterpart): the filled map as a "GMAPFNAV"
+    // byte string or file, loadable in place of FillMapManually.
+    std::vector<unsigned char> SaveNav() const {
+        long long n = 0;
+        check(gmapf_nav_encode(h_, nullptr, 0, &n));
+        std::vector<unsigned char> b((size_t)n);
+        check(gmapf_nav_encode(h_, b.data(), n, &n));
+        return b;
+    }
+    void SaveNav(const std::string& path) const { check(gmapf_save_nav(h_, path.c_str())); }
+    void FillMapFromNav(const std::vector<unsigned char>& bytes) {
+        check(gmapf_fill_map_from_nav(h_, bytes.data(), (long long)bytes.size()));
+    }
+    void LoadNav(const std::string& path) { check(gmapf_load_nav(h_, path.c_str())); }
 
     void AsyncStep(float dt) { check(gmapf_async_step(h_, dt)); }
     void WaitStepEnd() { check(gmapf_wait_step_end(h_)); }

@@ -80,6 +80,17 @@ int gmapf_fill_map_obstacle(gmapf_handle h, const float* xy, int n);   /* GMAPF.
 int gmapf_fill_map_end(gmapf_handle h);                                /* GMAPF.cu:161 */
 int gmapf_fill_map_manually(gmapf_handle h, const gmapf_node* nodes, int n); /* GMAPF.cu:199 */
 
+/* ---- map fixture (no reference counterpart): a filled navmesh as one
+ * versioned, checksummed byte string ("GMAPFNAV" v1: vertices, centroids,
+ * CSR neighbour lists), loadable in place of FillMapManually (same call order
+ * and errors as gmapf_fill_map_manually).  For callers that build a map once
+ * with the triangulator and load it many times. */
+/* bytes of the current map; buf NULL: only *len is set */
+int gmapf_nav_encode(gmapf_handle h, void* buf, long long cap, long long* len);
+int gmapf_fill_map_from_nav(gmapf_handle h, const void* buf, long long len);
+int gmapf_save_nav(gmapf_handle h, const char* path);
+int gmapf_load_nav(gmapf_handle h, const char* path);
+
 /* ---- agents (GMAPF.h:43-44) */
 int gmapf_add_agent(gmapf_handle h, float x, float y, int* id);        /* GMAPF.cu:219 */
 int gmapf_set_agent_targ_pos(gmapf_handle h, int id, float x, float y);/* GMAPF.cu:238 */

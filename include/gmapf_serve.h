@@ -11,15 +11,32 @@
  *   wait_step_end()                       -- WaitStepEnd on every map, retirement
  *   poll()                                -- completed requests
  *
- * Admission (at async_step, in arrival order): a request is admitted when
+ * Admission (at async_step): a request is admitted when
  *   - its agent has no request of the server in flight (requests for one
  *     agent run one after another), and
  *   - its map has a free search slot (PathFinderParallelAgents), and
  *   - the live search memory of all maps plus its footprint stays within
  *     queue_memory_budget.
- * A request held back by the first two conditions is skipped over; one held
- * back by the memory budget stops the scan, so a large-map request is never
- * overtaken indefinitely by small ones.  Footprint of a request on a map with
+ * A request held back by the first two conditions is skipped over.  The
+ * order of the scan is the admission policy:
+ *   GMAPF_ADMIT_FIFO (default): arrival order over all maps; a request held
+ *     back by the memory budget stops the scan, so a large-map request is
+ *     never overtaken indefinitely by small ones.  A burst on one map delays
+ *     every later request of the other maps.
+ *   GMAPF_ADMIT_FAIR: per-map arrival order, maps served by deficit round
+ *     robin over bytes: at each admission every map with queued requests
+ *     gains one quantum of credit (the largest footprint of all maps, so any
+ *     map can afford one request per frame) and the maps take turns, one
+ *     request per turn, from a start that rotates every frame, while their
+ *     credit covers their next request's footprint.  A map whose credit covers its next
+ *     request but the memory budget does not stops the scan (the budget
+ *     drains towards it), so no map starves.  Budget left after the fair
+ *     passes goes to the remaining requests in arrival order (charged to
+ *     their maps' credit), so admission never idles while memory is free.
+ *     Each map receives an equal share of the budget under contention,
+ *     whatever the others' arrival rates.  Credit is kept within
+ *     [-4, +2] quanta and reset when a map's queue empties.
+ * Footprint of a request on a map with
  * V nodes, T lanes and queue capacity Q:
  *   T * Q * 8 (parked queue keys) + T * 4 (queue counts) + VP (visited
  *   marks, one byte per node, V rounded up to 64) + V * 8 (G, direction and
@@ -56,7 +73,11 @@ typedef struct gmapf_server_config {
     int record_start_nodes;        /* nonzero: fill gmapf_request_done.start_node (reads all
                                       agents' nodes of a map from the device in every frame
                                       that admits on it) */
+    int admission_policy;          /* GMAPF_ADMIT_FIFO or GMAPF_ADMIT_FAIR */
 } gmapf_server_config;
+
+#define GMAPF_ADMIT_FIFO 0
+#define GMAPF_ADMIT_FAIR 1
 
 typedef struct gmapf_request_done {
     long long ticket;      /* returned by submit, increasing from 0 */

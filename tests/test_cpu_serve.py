@@ -119,3 +119,73 @@ def test_agents_searching_before_start_hold_slots():
     assert list(s.plan()) == [1]
     st = s.stats()
     assert st["in_flight"] == 2 and st["live_bytes"] == 2 * s.footprint(0)
+
+
+def _grid_engine(n_cells, P, n_agents, seed=3):
+    tri, nbr = H.grid_map(seed, n_cells, n_cells, cell=4.0, p=0.0)
+    cen = tri.reshape(-1, 3, 2).mean(axis=1)
+    g = G.GMAPF()
+    g.init(G.Config(agents_max_count=n_agents, path_finder_parallel_agents=P, path_finder_threads_per_agents=8,
+                    path_finder_queue_capacity=32, step_mode=G.MODE_BATCHED, round_budget=8))
+    g.fill_map_manually(tri, nbr)
+    for i in range(n_agents):
+        g.add_agent(*cen[i])
+    return g, cen
+
+
+def _flood(policy, budget_quanta=3, n_small=40, n_large=3):
+    from gmapf_amd.serving import ADMIT_FAIR  # noqa: F401
+    small, cs = _grid_engine(12, P=64, n_agents=64)
+    large, cl = _grid_engine(40, P=8, n_agents=8)
+    probe = Server([small, large])
+    fs, fl = probe.footprint(0), probe.footprint(1)
+    probe.close()
+    s = Server([small, large], queue_memory_budget=budget_quanta * fl if budget_quanta else 0,
+               admission_policy=policy)
+    small_t = [s.submit(0, a, *cs[-1 - a]) for a in range(n_small)]  # the burst arrives first
+    large_t = [s.submit(1, a, *cl[-1 - a]) for a in range(n_large)]
+    return s, fs, fl, small_t, large_t, (small, large)
+
+
+def test_fair_policy_admits_other_maps_during_a_burst():
+    from gmapf_amd.serving import ADMIT_FAIR, ADMIT_FIFO
+    s, fs, fl, small_t, large_t, keep = _flood(ADMIT_FIFO)
+    fifo = set(s.plan().tolist())
+    assert fifo and fifo <= set(small_t)  # the budget is spent on the burst, map 1 waits
+    s, fs, fl, small_t, large_t, keep = _flood(ADMIT_FAIR)
+    fair = s.plan().tolist()
+    assert fair == s.plan().tolist()  # a plan changes no state
+    assert set(fair) & set(large_t) == {large_t[0]}  # one quantum: one large request this frame
+    # deficit round robin, one request per turn, from map 0: map 0, map 1,
+    # then map 0 until its quantum is spent; the budget left goes to the
+    # burst in arrival order
+    q = max(fs, fl)
+    n0 = q // fs
+    assert fair[:2] == [small_t[0], large_t[0]] and fair[2:n0 + 1] == small_t[1:n0]
+    extra = (3 * fl - (n0 * fs + fl)) // fs
+    assert fair[n0 + 1:] == small_t[n0:n0 + extra]
+    assert sum(fs if t in small_t else fl for t in fair) <= 3 * fl
+
+
+def test_fair_policy_without_budget_admits_what_fifo_admits():
+    from gmapf_amd.serving import ADMIT_FAIR, ADMIT_FIFO
+    s, _, _, small_t, large_t, keep = _flood(ADMIT_FIFO, budget_quanta=0)
+    fifo = set(s.plan().tolist())
+    s, _, _, small_t, large_t, keep = _flood(ADMIT_FAIR, budget_quanta=0)
+    fair = set(s.plan().tolist())
+    assert fair == fifo == set(small_t) | set(large_t)
+
+
+def test_fair_policy_keeps_agents_serialised_and_slots_bounded():
+    from gmapf_amd.serving import ADMIT_FAIR
+    g, cen = _grid_engine(12, P=3, n_agents=8)
+    s = Server([g], admission_policy=ADMIT_FAIR)
+    t = [s.submit(0, 0, *cen[20]), s.submit(0, 0, *cen[21]), s.submit(0, 1, *cen[22]),
+         s.submit(0, 2, *cen[23]), s.submit(0, 3, *cen[24])]
+    assert s.plan().tolist() == [t[0], t[2], t[3]]  # agent 0 once; 3 slots
+
+
+def test_unknown_admission_policy_is_rejected():
+    g, _ = _grid_engine(12, P=3, n_agents=2)
+    with pytest.raises(G.GmapfError):
+        Server([g], admission_policy=5)

@@ -7,7 +7,7 @@ import pytest
 
 import gmapf_amd as G
 from gmapf_amd.runner import Locators
-from gmapf_amd.serving import Server, build_map, serving_maps
+from gmapf_amd.serving import ADMIT_FAIR, ADMIT_FIFO, Server, build_map, serving_maps
 
 pytestmark = pytest.mark.gpu
 
@@ -51,13 +51,14 @@ def _drive(srv, smaps, engines, frames, lam, seed, corridors):
     return np.concatenate(done_all)
 
 
-def test_results_alone_equal_results_in_the_mix(oracle):
+@pytest.mark.parametrize("policy", [ADMIT_FIFO, ADMIT_FAIR])
+def test_results_alone_equal_results_in_the_mix(oracle, policy):
     locs = Locators()
     smaps = _maps(locs)
     engines = [build_map(sm, T, Q, round_budget=3) for sm in smaps]
     fb = [(e.launch_info()["V"] + 63) // 64 * 64 + e.launch_info()["V"] * 8 + T * Q * 8 + T * 4 for e in engines]
     budget = 12 * fb[1] + 12 * fb[0]  # fewer than the slots: the budget binds
-    srv = Server(engines, queue_memory_budget=budget, record_start_nodes=True)
+    srv = Server(engines, queue_memory_budget=budget, record_start_nodes=True, admission_policy=policy)
     assert [srv.footprint(m) for m in range(2)] == fb
     log = {}
     # requests keep arriving while earlier ones are searching (round budget 3)
@@ -156,3 +157,58 @@ def test_three_maps_overlap_and_budget_bounds_concurrency():
     srv.close()
     for e in engines:
         e.close()
+
+
+def _burst_run(policy, smaps, n_small, n_large, budget):
+    engines = [build_map(sm, 8, 16, round_budget=16) for sm in smaps]
+    srv = Server(engines, queue_memory_budget=budget, admission_policy=policy)
+    # every agent gets one request: its start node is its initial position,
+    # so each request's search is the same under both policies
+    ms = np.array([0] * n_small + [1] * n_large, np.int32)
+    ag = np.array(list(range(n_small)) + list(range(n_large)), np.int32)
+    xy = np.stack([smaps[0].targets[i] for i in range(n_small)] + [smaps[1].targets[i] for i in range(n_large)])
+    t, st = srv.submit_many(ms, ag, xy)  # the burst on map 0 arrives first
+    assert (st == 0).all()
+    done = []
+    for f in range(3000):
+        srv.step(1.0 / 60)
+        done.append(srv.poll())
+        s = srv.stats()
+        if s["pending"] == 0 and s["in_flight"] == 0:
+            break
+    done = np.concatenate(done)
+    assert srv.stats()["peak_live_bytes"] <= budget
+    corr = {int(d["ticket"]): engines[int(d["map"])].corridor(int(d["agent"])) for d in done if d["found"]}
+    srv.close()
+    for e in engines:
+        e.close()
+    return done, corr
+
+
+def test_fair_policy_bounds_the_wait_of_a_map_behind_a_burst():
+    locs = Locators()
+    smaps = serving_maps(locs, seed=4, sizes=(160.0, 640.0), agents=(600, 32), slots=(600, 8),
+                         target_pool=4096)
+    probe = Server([build_map(sm, 8, 16, round_budget=16) for sm in smaps])
+    fl = probe.footprint(1)
+    probe.close()
+    budget = 3 * fl
+    print("burst fifo", flush=True)
+    fifo, cf = _burst_run(ADMIT_FIFO, smaps, 600, 8, budget)
+    print("burst fair", flush=True)
+    fair, ca = _burst_run(ADMIT_FAIR, smaps, 600, 8, budget)
+    assert len(fifo) == len(fair) == 608
+    # same requests, same results: the policy changes only when a search runs
+    for a, b in ((fifo, fair),):
+        ka = {int(d["ticket"]): int(d["found"]) for d in a}
+        kb = {int(d["ticket"]): int(d["found"]) for d in b}
+        assert ka == kb
+    assert cf.keys() == ca.keys() and all(np.array_equal(cf[k], ca[k]) for k in cf)
+    wait = lambda d: d["admit_frame"] - d["submit_frame"]  # noqa: E731
+    lf, la = fifo[fifo["map"] == 1], fair[fair["map"] == 1]
+    # FIFO: map 1's requests wait behind the whole burst; fair: map 1 is served
+    # in the first admission and every later one it can afford
+    assert la["admit_frame"].min() == 0 < lf["admit_frame"].min()
+    assert wait(la).max() < wait(lf).max()
+    print(f"map-1 admission wait, frames: fifo max {wait(lf).max()} mean {wait(lf).mean():.1f}; "
+          f"fair max {wait(la).max()} mean {wait(la).mean():.1f}", flush=True)
