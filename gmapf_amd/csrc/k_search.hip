@@ -65,13 +65,23 @@ __device__ __forceinline__ float dist2(float2 a, float2 b) {
     return x * x + y * y;
 }
 
-// ---- per-lane binary min-heap in LDS, "keep the Q smallest" (CuQueue.h:33-70).
-// Entry i of lane L is q[i * 64 + L]; row Q is the lane's spare entry, the
-// target of the writes of a lane that has nothing to move, so the sifts run
-// a fixed number of levels without branches and the wave leaves a sift as
-// soon as no lane moves (a ballot per level).  All lanes of the wave call
-// these functions; `on` selects the lanes that act.
+// ---- per-lane 4-ary min-heap in LDS, "keep the Q smallest" (CuQueue.h:33-70).
+// Entry i of lane L is q[i * 64 + L]; node i's children are 4i+1 .. 4i+4
+// (a 16-item heap is two levels deep).  Rows Q .. Q+3 are the lane's spare
+// entries: the target of the writes of a lane that has nothing to move and
+// the padding of a child group that runs past the heap, so the sifts run a
+// fixed number of levels without branches and the wave leaves a sift as soon
+// as no lane moves (a ballot per level).  All lanes of the wave call these
+// functions; `on` selects the lanes that act.
 constexpr unsigned long long kKeyNone = ~0ull;  // larger than every key
+constexpr int kSpareRows = 4;
+
+// levels below the root of a Q-item 4-ary heap
+__device__ __forceinline__ int heap_depth(int Q) {
+    int d = 0;
+    for (long long full = 1; full < Q; full = 4 * full + 1) ++d;
+    return d;
+}
 
 // Sift key k up from hole i.
 __device__ __forceinline__ void heap_sift_up(unsigned long long* q, int L, int Q, int depth, bool on, int i,
@@ -80,7 +90,7 @@ __device__ __forceinline__ void heap_sift_up(unsigned long long* q, int L, int Q
     for (int d = 0; d < depth; ++d) {
         if (__ballot(!done) == 0ull) break;
         const bool hp = i > 0;
-        const int p = (i - 1) >> 1;
+        const int p = (i - 1) >> 2;
         const unsigned long long pv = q[(hp ? p : Q) * kWave + L];
         const bool move = !done && hp && pv > k;
         q[(move ? i : Q) * kWave + L] = pv;
@@ -97,16 +107,22 @@ __device__ __forceinline__ void heap_sift_down(unsigned long long* q, int L, int
     bool done = !on;
     for (int d = 0; d < depth; ++d) {
         if (__ballot(!done) == 0ull) break;
-        const int l = 2 * i + 1, r = l + 1;
-        unsigned long long lv = q[(l < Q ? l : Q) * kWave + L];
-        unsigned long long rv = q[(r < Q ? r : Q) * kWave + L];
-        lv = l < n ? lv : kKeyNone;
-        rv = r < n ? rv : kKeyNone;
-        const bool ur = rv < lv;
-        const unsigned long long cv = ur ? rv : lv;
+        const int c0 = 4 * i + 1;
+        const int cb = c0 < Q ? c0 : Q;  // rows cb .. cb+3 lie within the heap and spare rows
+        const unsigned long long* r = q + cb * kWave + L;
+        unsigned long long v0 = r[0], v1 = r[kWave], v2 = r[2 * kWave], v3 = r[3 * kWave];
+        v0 = c0 < n ? v0 : kKeyNone;
+        v1 = c0 + 1 < n ? v1 : kKeyNone;
+        v2 = c0 + 2 < n ? v2 : kKeyNone;
+        v3 = c0 + 3 < n ? v3 : kKeyNone;
+        const bool b1 = v1 < v0, b3 = v3 < v2;
+        const unsigned long long m01 = b1 ? v1 : v0, m23 = b3 ? v3 : v2;
+        const bool b23 = m23 < m01;
+        const unsigned long long cv = b23 ? m23 : m01;
+        const int ci = c0 + (b23 ? (b3 ? 3 : 2) : (b1 ? 1 : 0));
         const bool move = !done && cv < k;
         q[(move ? i : Q) * kWave + L] = cv;
-        i = move ? (ur ? r : l) : i;
+        i = move ? ci : i;
         done = !move;
     }
     q[(on ? i : Q) * kWave + L] = k;
@@ -114,7 +130,8 @@ __device__ __forceinline__ void heap_sift_down(unsigned long long* q, int L, int
 
 // Push key k (lanes with k != kKeyNone) into a heap bounded to `cap` items:
 // below the bound it is added; at the bound it replaces the largest item (a
-// leaf, indices cap/2 .. cap-1) when smaller, else it is dropped.
+// leaf: indices from parent(cap - 1) + 1 to cap - 1) when smaller, else it is
+// dropped.
 __device__ __forceinline__ void heap_push(unsigned long long* q, int L, int Q, int depth, int cap, int& cnt,
                                           unsigned long long k) {
     const bool on = k != kKeyNone;
@@ -124,7 +141,8 @@ __device__ __forceinline__ void heap_push(unsigned long long* q, int L, int Q, i
     if (__ballot(on && full) != 0ull) {
         int mi = 0;
         unsigned long long mv = 0ull;  // with no leaf (cap 0) every key is dropped
-        for (int j = cap >> 1; j < cap; ++j) {
+        const int first_leaf = cap > 1 ? ((cap - 2) >> 2) + 1 : 0;
+        for (int j = first_leaf; j < cap; ++j) {
             const unsigned long long v = q[j * kWave + L];
             mi = v > mv ? j : mi;
             mv = v > mv ? v : mv;
@@ -229,7 +247,7 @@ __device__ __forceinline__ void walk_step(Walk& w, int side, const DevAgents& ag
 template <int T>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_search(DevMap map, DevAgents ag, DevSlots slots, DevFrame fr,
                                                SearchParams sp, int persistent) {
-    extern __shared__ unsigned long long q[];  // 64 * (Q + 1) keys: Q heap rows + the spare row
+    extern __shared__ unsigned long long q[];  // 64 * (Q + 4) keys: Q heap rows + the spare rows
     constexpr int RPW = kWave / T;
     constexpr int HB = hash_bits<T>();
     constexpr int HS = 1 << HB;
@@ -253,7 +271,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const unsigned long long smask = seg_mask<T>(sub);
     const unsigned long long below = smask & ((1ull << L) - 1ull);
     const int Q = sp.Q;
-    const int depth = 31 - __clz(Q);  // levels below the root of a Q-item heap
+    const int depth = heap_depth(Q);
     const size_t V = (size_t)map.V;
     const int n_active = fr.counters[1];
 
@@ -555,14 +573,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (shrink) --cnt;
             const unsigned long long last = q[(shrink ? cnt : Q) * kWave + L];
             heap_sift_down(q, L, Q, depth, repl || (shrink && cnt > 0), cnt, repl ? d0 : last);
-            // the items still to push: the dealt items other than m (and d0
-            // when it replaced the root)
-            const unsigned long long p0 = (repl || m == d0) ? kKeyNone : d0;
-            const unsigned long long p1 = m == d1 ? kKeyNone : d1;
-            const unsigned long long p2 = m == d2 ? kKeyNone : d2;
-            heap_push(q, L, Q, depth, Q - 1, cnt, p0);
-            heap_push(q, L, Q, depth, Q - 1, cnt, p1);
-            heap_push(q, L, Q, depth, Q - 1, cnt, p2);
+            // the items still to push, at most two: the dealt items other
+            // than m (and d0 when it replaced the root)
+            const bool skip0 = repl || m == d0;
+            const unsigned long long pa = skip0 ? d1 : d0;
+            const unsigned long long pb = (skip0 || m == d1) ? d2 : d1;
+            heap_push(q, L, Q, depth, Q - 1, cnt, pa == m ? kKeyNone : pa);
+            heap_push(q, L, Q, depth, Q - 1, cnt, pb == m ? kKeyNone : pb);
         }
         popped_next = has_next;
         c_next = pf_c;
@@ -670,7 +687,7 @@ const void* search_fn(int T) {
 
 }  // namespace
 
-int search_lds_bytes(int Q) { return kWave * (Q + 1) * (int)sizeof(unsigned long long); }
+int search_lds_bytes(int Q) { return kWave * (Q + kSpareRows) * (int)sizeof(unsigned long long); }
 
 hipError_t search_prepare(int T, int Q) {
     const void* fn = search_fn(T);
