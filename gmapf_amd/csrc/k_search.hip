@@ -258,6 +258,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // the three slots' operations issue together without branches; it never
     // holds kHashEmpty, so a compare-and-swap on it never writes
     __shared__ int hdummy[kWave];
+    __shared__ int hsink[kWave];  // per-lane target of the table resets of an empty slot
     for (int i = threadIdx.x; i < RPW * HS; i += kWave) {
         hkey[i] = kHashEmpty;
         hval[i] = kHashNone;
@@ -303,8 +304,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     unsigned st_found = 0, st_fail = 0, st_exp = 0, st_disc = 0, st_corr = 0, st_rounds = 0;
 
     while (true) {
-        // ---- acquire work for segments that have none
-        if (!has_req && !no_more) {
+        // ---- acquire work for segments that have none (one wave-uniform
+        // test when every segment is busy; rare paths of the round are
+        // skipped the same way, so a round pays no lane-mask bookkeeping for
+        // them)
+        if (__ballot(!has_req && !no_more) != 0ull && !has_req && !no_more) {
             int item = -1;
             if (persistent) {
                 if (l == 0) item = atomicAdd(&fr.counters[2], 1);
@@ -379,53 +383,46 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             need_pop = false;
         }
         const bool popped = has_req && popped_next;
-        int c = -1, G = 0, cdir = 0;
-        int nn[3] = {-1, -1, -1};
-        float2 cnb[3];
-        bool have_vn = false;
-        if (popped) {
-            c = c_next;
-            int4 r0 = pf_r0, r1 = pf_r1;
-            int2 r2 = pf_r2;
-            uint2 vc = pf_vc;
-            have_vn = pf_vn_ok && c == pf_c;
-            if (c != pf_c) {
-                r0 = map.rec[3 * (size_t)c];
-                r1 = map.rec[3 * (size_t)c + 1];
-                r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)c + 2);
-                vc = vis[c];
-            }
-            G = (int)(vc.x >> 1);
-            cdir = (int)(vc.x & 1u);
-            nn[0] = r0.x;
-            nn[1] = nn[0] >= 0 ? r0.y : -1;
-            nn[2] = nn[1] >= 0 ? r0.z : -1;
-            cnb[0] = make_float2(__int_as_float(r1.x), __int_as_float(r1.y));
-            cnb[1] = make_float2(__int_as_float(r1.z), __int_as_float(r1.w));
-            cnb[2] = make_float2(__int_as_float(r2.x), __int_as_float(r2.y));
+        const int c = popped ? c_next : -1;
+        const bool miss = popped && c != pf_c;  // first round of a search: nothing prefetched
+        const bool have_vn = popped && pf_vn_ok && !miss;
+        int4 r0 = pf_r0, r1 = pf_r1;
+        int2 r2 = pf_r2;
+        uint2 vc = pf_vc;
+        if (__ballot(miss) != 0ull && miss) {
+            r0 = map.rec[3 * (size_t)c];
+            r1 = map.rec[3 * (size_t)c + 1];
+            r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)c + 2);
+            vc = vis[c];
         }
+        const int G = (int)(vc.x >> 1);
+        const int cdir = (int)(vc.x & 1u);
+        int nn[3];
+        nn[0] = popped ? r0.x : -1;
+        nn[1] = nn[0] >= 0 ? r0.y : -1;
+        nn[2] = nn[1] >= 0 ? r0.z : -1;
+        const float2 cnb[3] = {make_float2(__int_as_float(r1.x), __int_as_float(r1.y)),
+                               make_float2(__int_as_float(r1.z), __int_as_float(r1.w)),
+                               make_float2(__int_as_float(r2.x), __int_as_float(r2.y))};
         // marks of the neighbours: prefetched, or read now (all three loads
         // issued before the first is used)
         unsigned mk[3] = {pf_vn[0], pf_vn[1], pf_vn[2]};
-        if (popped && !have_vn) {
+        if (__ballot(popped && !have_vn) != 0ull && popped && !have_vn) {
 #pragma unroll
             for (int s = 0; s < 3; ++s) mk[s] = mark[nn[s] >= 0 ? nn[s] : 0];
         }
+        // F of every slot (a missing neighbour's value is not used); h is
+        // measured to the opposite endpoint of the popped node's direction
+        const float2 cfar = cdir ? cT : cS;
         float Fn[3];
         bool prior[3];
         int pdir[3];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-            Fn[s] = 0.f;
-            prior[s] = false;
-            pdir[s] = 0;
-            if (nn[s] >= 0) {
-                const float2 cn = cnb[s];
-                const float h = cdir ? dist2(cn, cT) : dist2(cn, cS);
-                Fn[s] = (float)(G + 1) + h * sp.K;
-                prior[s] = (mk[s] >> 1) == mval;
-                pdir[s] = (int)(mk[s] & 1u);
-            }
+            const float h = dist2(cnb[s], cfar);
+            Fn[s] = (float)(G + 1) + h * sp.K;
+            prior[s] = nn[s] >= 0 && (mk[s] >> 1) == mval;
+            pdir[s] = (int)(mk[s] & 1u);
         }
 
         // ---- this round's candidates on unmarked nodes: place each node in
@@ -450,7 +447,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             for (int s = 0; s < 3; ++s) pend[s] = ins[s] && old[s] != kHashEmpty && old[s] != nn[s];
         }
         // slot taken by another node: probe on (rare)
-        while (pend[0] || pend[1] || pend[2]) {
+        while (__ballot(pend[0] || pend[1] || pend[2]) != 0ull) {
 #pragma unroll
             for (int s = 0; s < 3; ++s) {
                 if (pend[s]) {
@@ -478,19 +475,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         bool claim[3], meet[3];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-            claim[s] = false;
-            meet[s] = false;
-            if (nn[s] < 0) continue;
-            int odir = pdir[s];
-            if (ins[s]) {
-                const int e = ev[s];
-                if ((e >> 1) == l * 3 + s) {
-                    claim[s] = true;
-                    continue;
-                }
-                odir = e & 1;
-            }
-            meet[s] = (odir != cdir);
+            claim[s] = ins[s] && (ev[s] >> 1) == l * 3 + s;
+            const int odir = ins[s] ? (ev[s] & 1) : pdir[s];
+            meet[s] = nn[s] >= 0 && !claim[s] && odir != cdir;
         }
 
         // ---- claims: frontier order = (lane, slot) order
@@ -530,10 +517,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // empty this round's table slots for the next round
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-            if (ins[s]) {
-                hk[hslot[s]] = kHashEmpty;
-                hv[hslot[s]] = kHashNone;
-            }
+            *(ins[s] ? &hk[hslot[s]] : &hsink[L]) = kHashEmpty;
+            *(ins[s] ? &hv[hslot[s]] : &hsink[L]) = kHashNone;
         }
 
         // ---- deal frontier item f to lane f mod T (at most three per lane),
@@ -570,7 +555,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             // the root's replacement: the first dealt item, or the last item
             const bool repl = from_top && d0 != kKeyNone;
             const bool shrink = from_top && !repl;
-            if (shrink) --cnt;
+            cnt -= shrink ? 1 : 0;
             const unsigned long long last = q[(shrink ? cnt : Q) * kWave + L];
             heap_sift_down(q, L, Q, depth, repl || (shrink && cnt > 0), cnt, repl ? d0 : last);
             // the items still to push, at most two: the dealt items other
@@ -595,19 +580,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         // one parent step of a pending corridor walk; its load was issued a
         // round ago
-        if (l < 2) walk_step(walk, l, ag, slots, V, st_corr);
+        if (__ballot(walk.active) != 0ull && l < 2) walk_step(walk, l, ag, slots, V, st_corr);
         const unsigned long long nb = __ballot(has_next) & smask;
-        if (has_req) {
-            ++rounds_here;
-            exp_here += __popcll(pb);
-            disc_here += tot;
-            if (found) {
-                status = 1;
-            } else if (nb == 0ull) {
-                status = 2;
-            }
-            const bool ended = status != 0;
-            const bool park = !ended && sp.budget > 0 && rounds_here >= sp.budget;
+        rounds_here += has_req ? 1 : 0;
+        exp_here += has_req ? __popcll(pb) : 0;
+        disc_here += has_req ? tot : 0;
+        status = !has_req ? status : (found ? 1 : (nb == 0ull ? 2 : status));
+        const bool ended = has_req && status != 0;
+        const bool park = has_req && !ended && sp.budget > 0 && rounds_here >= sp.budget;
+        if (__ballot(ended || park) != 0ull && (ended || park)) {
             if (park) {
                 // [node to expand next or kKeyNone, heap items]: at most Q keys
                 const size_t qb = ((size_t)slot * T + l);
@@ -647,7 +628,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     ag.inflight[agent] = -1;
                 }
             }
-            if (ended || park) has_req = false;
+            has_req = false;
         }
     }
     // walks still pending when the last search of the workgroup ended: the
