@@ -107,6 +107,7 @@ struct DevFrame {
                      // [4] n_free [5] moving count [6] n_admitted
                      // [7] blocks done in the admission count pass
                      // [8] slots whose marks wrap this frame
+                     // [9] ticket stride: ticket t takes active[(t * stride) % n_active]
     int* clear_list; // P: those slots
     unsigned long long* stats;  // [0] found [1] fail [2] expansions [3] discoveries
                                 // [4] corridor nodes [5] rounds [6] admitted

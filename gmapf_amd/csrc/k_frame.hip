@@ -297,6 +297,26 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_count(DevAgents ag, Dev
         fr.counters[6] = admitted;
         fr.counters[7] = 0;  // every block has counted itself: ready for the next frame
         fr.counters[8] = 0;
+        // Searches still running from earlier frames (the long ones) come
+        // first in the active list and new ones last; a stride coprime with
+        // the count, near 0.618 of it, deals both kinds to every workgroup,
+        // so no workgroup holds only long searches (a launch lasts as long
+        // as its slowest workgroup).
+        const int na = fr.counters[1];
+        int stride = 1;
+        if (na > 2) {
+            stride = (int)(((long long)na * 40503) >> 16) | 1;
+            for (;; stride += 2) {
+                int a = na, b = stride;
+                while (b) {
+                    const int t = a % b;
+                    a = b;
+                    b = t;
+                }
+                if (a == 1) break;
+            }
+        }
+        fr.counters[9] = stride;
         fr.stats[6] += (unsigned long long)admitted;
     }
 }

@@ -275,6 +275,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int depth = heap_depth(Q);
     const size_t V = (size_t)map.V;
     const int n_active = fr.counters[1];
+    const long long stride = fr.counters[9];
 
     // per-segment search state (replicated in every lane of the segment)
     int slot = -1, agent = -1;
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (item < 0 || item >= n_active) {
                 no_more = true;
             } else {
-                slot = fr.active[item];
+                slot = fr.active[(int)(((long long)item * stride) % n_active)];
                 agent = slots.agent[slot];
                 S = slots.S[slot];
                 Tg = slots.Tg[slot];
