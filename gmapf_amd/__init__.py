@@ -60,6 +60,7 @@ class _Config(C.Structure):
         ("rot_cs", C.c_float),
         ("rot_sn", C.c_float),
         ("device", C.c_int),
+        ("corridor_slab", C.c_longlong),
     ]
 
 
@@ -91,6 +92,7 @@ class Config:
     rot_cs: float = 0.0
     rot_sn: float = 0.0
     device: int = 0
+    corridor_slab: int = 0
 
     def _to_c(self) -> _Config:
         c = _Config()
@@ -137,6 +139,7 @@ _SIGS = {
     "gmapf_last_frame_ms": [C.c_void_p, C.c_void_p],
     "gmapf_get_hash": [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)],
     "gmapf_get_launch_info": [C.c_void_p, C.c_void_p],
+    "gmapf_get_slab_info": [C.c_void_p, C.c_void_p],
     "gmapf_get_config": [C.c_void_p, C.POINTER(_Config)],
     "gmapf_agent_states": [C.c_void_p, C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)],
     # serving loop (include/gmapf_serve.h)
@@ -395,6 +398,11 @@ class GMAPF:
         tot = C.c_int()
         self._check(self._lib.gmapf_get_hash(self._h, _ptr(keys), _ptr(order), C.byref(tot)))
         return keys[:n], order[:tot.value]
+
+    def slab_info(self):
+        out = (C.c_longlong * 4)()
+        self._check(self._lib.gmapf_get_slab_info(self._h, out))
+        return dict(capacity=out[0], in_use=out[1], compactions=out[2], deferred=out[3])
 
     def launch_info(self):
         out = np.zeros(8, dtype=np.int32)

@@ -38,7 +38,7 @@ TUNE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json
 T_SET = (2, 4, 8, 16, 32, 64)
 Q_SET = (16, 32, 64, 128, 256)
 LDS_BYTES = 160 * 1024
-BUDGETS = (48, 64, 96, 128, 192, 256)
+BUDGETS = (64, 96, 128, 192, 256, 384, 512)
 
 
 def lds_fits(Q):

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -80,11 +81,17 @@ struct gmapf_s {
     float* h_pos_stage = nullptr;
     int* h_state_stage = nullptr;
     unsigned long long* h_stats = nullptr;
+    unsigned long long* h_slab = nullptr;  // [0] slab ints in use, [1] deferred corridors
     int* h_upd = nullptr;       // [0] count, then ids[N], targ[N]
     float2* h_upd_xy = nullptr;
     int upd_cap = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // the batched frame with corridor-slab compaction before the search
+    hipGraph_t graph_c = nullptr;
+    hipGraphExec_t gexec_c = nullptr;
+    bool compact_next = false;   // the slab was half full after the last frame
+    long long compactions = 0, slab_in_use = 0, slab_deferred = 0;
 
     // profiling
     Clock::time_point t_step;
@@ -118,19 +125,25 @@ struct gmapf_s {
         if (s2) hipStreamSynchronize(s2);
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
+        if (gexec_c) hipGraphExecDestroy(gexec_c);
+        if (graph_c) hipGraphDestroy(graph_c);
         gexec = nullptr;
         graph = nullptr;
+        gexec_c = nullptr;
+        graph_c = nullptr;
         for (void* p : dallocs) hipFree(p);
         dallocs.clear();
         if (h_step) hipHostFree(h_step);
         if (h_pos_stage) hipHostFree(h_pos_stage);
         if (h_state_stage) hipHostFree(h_state_stage);
         if (h_stats) hipHostFree(h_stats);
+        if (h_slab) hipHostFree(h_slab);
         if (h_upd) hipHostFree(h_upd);
         if (h_upd_xy) hipHostFree(h_upd_xy);
         h_step = h_pos_stage = nullptr;
         h_state_stage = nullptr;
         h_stats = nullptr;
+        h_slab = nullptr;
         h_upd = nullptr;
         h_upd_xy = nullptr;
         for (auto& e : ev_p)
@@ -312,8 +325,23 @@ struct gmapf_s {
         dag.step_node = dalloc<int>(Nn, "agent step node");
         dag.step_pos = dalloc<float2>(Nn, "agent step pos");
         dag.path_len = dalloc<int>(Nn, "agent path len");
-        dag.path_cap = V;  // a corridor visits each node at most once
-        dag.path = dalloc<int>((size_t)Nn * V, "agent corridors");
+        dag.path_off = dalloc<long long>(Nn, "agent corridor offsets");
+        {
+            // a corridor visits each node at most once, so V ints always
+            // hold one; N * V hold every agent's
+            const long long dense = (long long)Nn * V;
+            long long cap = cfg.corridor_slab > 0 ? cfg.corridor_slab
+                                                   : std::min(dense, std::max((long long)Nn * 1024, 16LL * V));
+            cap = std::min(std::max(cap, (long long)V), (long long)INT_MAX - 1);  // offsets are 32-bit
+            dag.slab_cap = cap;
+            dag.slab[0] = dalloc<int>((size_t)cap, "corridor slab");
+            dag.slab[1] = dalloc<int>((size_t)cap, "corridor slab (compaction copy)");
+            dag.slab_now = dalloc<int*>(1, "corridor slab in use");
+            dag.slab_top = dalloc<unsigned long long>(2, "corridor slab top");
+            dag.slab_deferred = dag.slab_top + 1;
+            HIPCHK(hipMemcpy(dag.slab_now, &dag.slab[0], sizeof(int*), hipMemcpyHostToDevice));
+            HIPCHK(launch_fill_u64(dag.slab_top, 0, 2, s1));
+        }
         dag.gen = dalloc<int>(Nn, "agent gen");
         dag.inflight = dalloc<int>(Nn, "agent inflight");
         dag.in_m = dalloc<int>(Nn, "agent moving flag");
@@ -330,6 +358,7 @@ struct gmapf_s {
         HIPCHK(launch_fill(dag.step_idx, -1, Nn, s1));
         HIPCHK(launch_fill(dag.step_node, -1, Nn, s1));
         HIPCHK(launch_fill(dag.path_len, 0, Nn, s1));
+        HIPCHK(hipMemsetAsync(dag.path_off, 0, Nn * sizeof(long long), s1));
         HIPCHK(launch_fill(dag.inflight, -1, Nn, s1));
         HIPCHK(launch_fill(dag.in_m, 0, Nn, s1));
         HIPCHK(hipMemsetAsync(dag.step_pos, 0, Nn * sizeof(float2), s1));
@@ -343,6 +372,8 @@ struct gmapf_s {
         dsl.Tg = dalloc<int>(P, "slot target");
         dsl.stamp = dalloc<unsigned>(P, "slot stamp");
         dsl.status = dalloc<int>(P, "slot status");
+        dsl.mS = dalloc<int>(P, "slot meet start side");
+        dsl.mT = dalloc<int>(P, "slot meet target side");
         dsl.VP = ((long long)V + 63) / 64 * 64;
         dsl.mark = dalloc<unsigned char>((size_t)P * dsl.VP, "visited marks");
         dsl.vis = dalloc<uint2>((size_t)P * V, "visited entries");
@@ -378,6 +409,8 @@ struct gmapf_s {
         dfr.hscatter = dalloc<int>(Nn, "hash scatter");
         dfr.horder = dalloc<int>(Nn, "hash order");
         dfr.hblk = dalloc<int>(dfr.ncell / scan_block + 2, "hash scan partials");
+        dfr.slab_len = dalloc<int>((size_t)Nn + 1, "corridor lengths");
+        dfr.slab_blk = dalloc<int>(Nn / scan_block + 2, "corridor scan partials");
         dfr.stepv = dalloc<float>(1, "step");
         HIPCHK(launch_fill(dfr.counters, 0, 16, s1));
         HIPCHK(launch_fill_u64(dfr.stats, 0, 8, s1));
@@ -388,6 +421,8 @@ struct gmapf_s {
         HIPCHK(hipHostMalloc((void**)&h_pos_stage, Nn * sizeof(float2)));
         HIPCHK(hipHostMalloc((void**)&h_state_stage, Nn * sizeof(int)));
         HIPCHK(hipHostMalloc((void**)&h_stats, 8 * sizeof(unsigned long long)));
+        HIPCHK(hipHostMalloc((void**)&h_slab, 2 * sizeof(unsigned long long)));
+        h_slab[0] = h_slab[1] = 0;
         upd_cap = std::max(N, 1);
         HIPCHK(hipHostMalloc((void**)&h_upd, (1 + 2 * (size_t)upd_cap) * sizeof(int)));
         HIPCHK(hipHostMalloc((void**)&h_upd_xy, (size_t)upd_cap * sizeof(float2)));
@@ -401,6 +436,7 @@ struct gmapf_s {
         dfr.h_pos_out = reinterpret_cast<float2*>(h_pos_stage);
         dfr.h_state_out = h_state_stage;
         dfr.h_stats_out = h_stats;
+        dfr.h_slab_out = h_slab;
 
         // ---- search launch shape
         sp = SearchParams{T, Q, cfg.path_finder_heuristic_k, 0};
@@ -424,8 +460,9 @@ struct gmapf_s {
     }
 
     // ------------------------------------------------------------ frame
-    void enqueue_frame() {
+    void enqueue_frame(bool compact) {
         HIPCHK(launch_admit(dmap, dag, dsl, dfr, s1));
+        if (compact) HIPCHK(launch_compact(dag, dfr, s1));
         HIPCHK(hipEventRecord(ev_fork, s1));
         HIPCHK(hipStreamWaitEvent(s2, ev_fork, 0));
         HIPCHK(launch_search(dmap, dag, dsl, dfr, sp, search_grid, persistent, s1));
@@ -443,9 +480,10 @@ struct gmapf_s {
             throw ApiError(GMAPF_E_DEVICE, std::string("hip error after ") + phase + ": " + hipGetErrorString(e));
     }
 
-    void enqueue_frame_profiled() {
+    void enqueue_frame_profiled(bool compact) {
         HIPCHK(hipEventRecord(ev_p[0], s1));
         HIPCHK(launch_admit(dmap, dag, dsl, dfr, s1));
+        if (compact) HIPCHK(launch_compact(dag, dfr, s1));
         if (cfg.is_debug) debug_sync("admit");
         HIPCHK(hipEventRecord(ev_p[1], s1));
         HIPCHK(launch_search(dmap, dag, dsl, dfr, sp, search_grid, persistent, s1));
@@ -481,25 +519,32 @@ struct gmapf_s {
             upd_targ.clear();
             upd_xy.clear();
         }
+        // compact the corridor slab before this frame's search when the
+        // last frame left it half full (the host reads the slab top that the
+        // last frame exported; no device round trip)
+        const bool compact = compact_next;
+        if (compact) compactions++;
         if (cfg.step_mode == GMAPF_MODE_BATCHED && !cfg.is_profiler && !cfg.is_debug) {
-            if (!gexec) {
+            hipGraph_t& gr = compact ? graph_c : graph;
+            hipGraphExec_t& ge = compact ? gexec_c : gexec;
+            if (!ge) {
                 HIPCHK(hipStreamBeginCapture(s1, hipStreamCaptureModeThreadLocal));
                 try {
-                    enqueue_frame();
+                    enqueue_frame(compact);
                 } catch (...) {
                     hipGraph_t g = nullptr;
                     hipStreamEndCapture(s1, &g);
                     if (g) hipGraphDestroy(g);
                     throw;
                 }
-                HIPCHK(hipStreamEndCapture(s1, &graph));
-                HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+                HIPCHK(hipStreamEndCapture(s1, &gr));
+                HIPCHK(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
             }
-            HIPCHK(hipGraphLaunch(gexec, s1));
+            HIPCHK(hipGraphLaunch(ge, s1));
         } else if (cfg.is_profiler || cfg.is_debug) {
-            enqueue_frame_profiled();
+            enqueue_frame_profiled(compact);
         } else {
-            enqueue_frame();
+            enqueue_frame(compact);
         }
         HIPCHK(hipEventRecord(ev_done, s1));
         t_enq = Clock::now();
@@ -524,6 +569,9 @@ struct gmapf_s {
             std::memcpy(state.data(), h_state_stage, (size_t)n_agents * sizeof(int));
         }
         for (int k = 0; k < 8; k++) stats_host[k] = (long long)h_stats[k];
+        slab_in_use = (long long)h_slab[0];
+        slab_deferred = (long long)h_slab[1];
+        compact_next = 2 * slab_in_use > dag.slab_cap;
         frames++;
         stats_host[7] = frames;
         // target changes made while the step ran stay pending for the next step
@@ -891,17 +939,22 @@ int gmapf_get_agent_corridor(gmapf_handle h, int id, int* buf, int cap, int* len
             return;
         }
         if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
-        int L = 0, st = 0;
+        int L = 0;
+        long long off = 0;
+        int* slab = nullptr;
         HIPCHK(hipMemcpy(&L, h->dag.path_len + id, sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(&st, h->dag.state + id, sizeof(int), hipMemcpyDeviceToHost));
         if (L <= 0) {
             if (len) *len = -1;
             return;
         }
+        HIPCHK(hipMemcpy(&off, h->dag.path_off + id, sizeof(long long), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&slab, h->dag.slab_now, sizeof(int*), hipMemcpyDeviceToHost));
+        if (slab != h->dag.slab[0] && slab != h->dag.slab[1])
+            throw ApiError(GMAPF_E_INTERNAL, "corridor slab pointer is corrupt");
+        if (off < 0 || off + L > h->dag.slab_cap) throw ApiError(GMAPF_E_INTERNAL, "corridor outside the slab");
         if (len) *len = L;
         if (buf && cap > 0)
-            HIPCHK(hipMemcpy(buf, h->dag.path + (size_t)id * h->dag.path_cap,
-                             (size_t)std::min(cap, L) * sizeof(int), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(buf, slab + off, (size_t)std::min(cap, L) * sizeof(int), hipMemcpyDeviceToHost));
     });
 }
 
@@ -1059,6 +1112,17 @@ int gmapf_get_hash(gmapf_handle h, int* keys, int* order, int* n) {
         if (keys && N) HIPCHK(hipMemcpy(keys, h->dfr.hkey, N * sizeof(int), hipMemcpyDeviceToHost));
         if (order && total > 0) HIPCHK(hipMemcpy(order, h->dfr.horder, total * sizeof(int), hipMemcpyDeviceToHost));
         if (n) *n = total;
+    });
+}
+
+int gmapf_get_slab_info(gmapf_handle h, long long* out4) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!out4) throw ApiError(GMAPF_E_ARG, "null output");
+        out4[0] = h->started ? h->dag.slab_cap : 0;
+        out4[1] = h->slab_in_use;
+        out4[2] = h->compactions;
+        out4[3] = h->slab_deferred;
     });
 }
 

@@ -14,9 +14,16 @@
 //   plus a short list of near-degenerate triangles that is always scanned.
 // Agents (SoA, one entry per agent id):
 //   pos float2, curr int, targ_pos float2, targ int, state int,
-//   step_idx int, step_node int, step_pos float2, path_len int,
-//   path[id * path_cap ...] corridor, gen int (bumped by SetAgentTargPos),
-//   inflight int (search slot or -1), in_m int (moving set of this frame).
+//   step_idx int, step_node int, step_pos float2, path_len int, path_off
+//   long long (corridor = slab[path_off .. path_off + path_len)), gen int
+//   (bumped by SetAgentTargPos), inflight int (search slot or -1), in_m int
+//   (moving set of this frame).
+// Corridor slab: two buffers of slab_cap ints; *slab_now is the one in use.
+//   A found search takes path_len ints at *slab_top (atomic bump); a full
+//   slab defers the corridor (the search keeps its slot, status 4, and
+//   retries next frame).  Compaction (when the host sees the slab half
+//   full) copies the corridors of moving agents into the other buffer in
+//   agent order and drops the rest.
 // Search slots (P of them, PathFinderParallelAgents):
 //   mark[slot * VP + node] byte (m << 1) | dir, m = stamp % 127 + 1 of the
 //   search that claimed the node: the probe a search makes for every
@@ -67,8 +74,12 @@ struct DevAgents {
     int* step_node;
     float2* step_pos;
     int* path_len;
-    int* path;
-    long long path_cap;
+    long long* path_off;
+    int** slab_now;                // device word: the slab buffer in use
+    int* slab[2];
+    long long slab_cap;            // ints per buffer
+    unsigned long long* slab_top;  // ints allocated in the buffer in use
+    unsigned long long* slab_deferred;  // corridors deferred for lack of slab space (total)
     int* gen;
     int* inflight;
     int* in_m;
@@ -82,7 +93,10 @@ struct DevSlots {
     int* S;
     int* Tg;
     unsigned* stamp;
-    int* status;    // 0 running, 1 found, 2 exhausted, 3 fresh (needs seeding)
+    int* status;    // 0 running, 1 found, 2 exhausted, 3 fresh (needs seeding),
+                    // 4 found, corridor deferred (needs slab space)
+    int* mS;        // meet of a found search: start-side and target-side nodes
+    int* mT;
     unsigned char* mark;    // P * VP  (m << 1) | dir
     uint2* vis;             // P * V   {G << 1 | dir, parent}
     long long VP;           // mark stride: V rounded up to 64
@@ -131,6 +145,10 @@ struct DevFrame {
     float2* h_pos_out;           // [N] positions after the frame
     int* h_state_out;            // [N] states after the frame
     unsigned long long* h_stats_out;  // [8] cumulative counters
+    unsigned long long* h_slab_out;   // [2] slab top, deferred corridors
+    // corridor slab compaction scratch
+    int* slab_len;                    // N + 1: live corridor lengths, then their offsets
+    int* slab_blk;                    // scan partials
 };
 
 struct SearchParams {

@@ -406,6 +406,42 @@ __global__ void k_export(DevAgents ag, DevFrame fr) {
         fr.h_state_out[i] = ag.state[i];
     }
     if (i < 8) fr.h_stats_out[i] = fr.stats[i];
+    if (i < 2) fr.h_slab_out[i] = ag.slab_top[i];  // [0] top, [1] deferred (adjacent words)
+}
+
+// ------------------------------------------------------------- corridor slab
+// live length per agent: a moving agent keeps its corridor, the others' are
+// released
+__global__ void k_slab_len(DevAgents ag, DevFrame fr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ag.n) fr.slab_len[i] = (ag.state[i] == kMove && ag.path_len[i] > 0) ? ag.path_len[i] : 0;
+}
+
+// after the scan slab_len[i] is agent i's new offset and slab_len[n] the
+// total; one wave copies one agent's corridor
+__global__ void k_slab_copy(DevAgents ag, DevFrame fr) {
+    const int* src = *ag.slab_now;
+    int* dst = src == ag.slab[0] ? ag.slab[1] : ag.slab[0];
+    const int lane = threadIdx.x % 64;
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int nw = (gridDim.x * blockDim.x) / 64;
+    for (int a = w; a < ag.n; a += nw) {
+        const int off = fr.slab_len[a];
+        const int len = fr.slab_len[a + 1] - off;
+        const long long from = ag.path_off[a];
+        for (int k = lane; k < len; k += 64) dst[off + k] = src[from + k];
+        if (lane == 0) {
+            ag.path_off[a] = off;
+            ag.path_len[a] = len;
+        }
+    }
+}
+
+__global__ void k_slab_finish(DevAgents ag, DevFrame fr) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int* cur = *ag.slab_now;
+    *ag.slab_now = cur == ag.slab[0] ? ag.slab[1] : ag.slab[0];
+    *ag.slab_top = (unsigned long long)fr.slab_len[ag.n];
 }
 
 // ------------------------------------------------------------- crowd
@@ -524,7 +560,7 @@ __global__ void k_progress(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp
     const float2 p = ag.pos_tmp[i];
     ag.pos[i] = p;
     if (!ag.in_m[i]) return;
-    const int* path = ag.path + (size_t)i * ag.path_cap;
+    const int* path = *ag.slab_now + ag.path_off[i];
     int idx = ag.step_idx[i];
     if (idx == -1) {
         const int n = path[0];
@@ -588,6 +624,16 @@ hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& 
 
 hipError_t launch_export(const DevAgents& ag, const DevFrame& fr, hipStream_t st) {
     hipLaunchKernelGGL(k_export, dim3(nblk(ag.n > 8 ? ag.n : 8, 256)), dim3(256), 0, st, ag, fr);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(const DevAgents& ag, const DevFrame& fr, hipStream_t st) {
+    if (ag.n <= 0) return hipSuccess;
+    hipError_t e;
+    hipLaunchKernelGGL(k_slab_len, dim3(nblk(ag.n, 256)), dim3(256), 0, st, ag, fr);
+    if ((e = exscan(fr.slab_len, ag.n, fr.slab_blk, fr.slab_len, &fr.slab_len[ag.n], st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_slab_copy, dim3(nblk(ag.n, 4) < 2048 ? nblk(ag.n, 4) : 2048), dim3(256), 0, st, ag, fr);
+    hipLaunchKernelGGL(k_slab_finish, dim3(1), dim3(64), 0, st, ag, fr);
     return hipGetLastError();
 }
 

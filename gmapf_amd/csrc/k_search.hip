@@ -195,51 +195,87 @@ __device__ __forceinline__ unsigned claim_hash(int n, int bits) {
 
 // Corridor walk of one side of a found search (PathFinder.h:276-306), one
 // parent step per call so the walk of a search that ended overlaps the rounds
-// of the other searches of the workgroup.  Lane 0 of the segment walks the
+// of the other searches of the workgroup.  The corridor occupies
+// slab[base .. base + G(mS) + G(mT) + 1).  Lane 0 of the segment walks the
 // start side from mS down to the start node (exclusive), writing indices
 // G(mS)-1 .. 0; lane 1 walks the target side from mT to the target node
 // (inclusive), writing indices G(mS) .. G(mS)+G(mT).  `ent` holds the visited
-// entry of `n`, requested by the previous step; lane 1 also requested mS's
-// entry, whose first word (`gS`) gives the start side's length.
+// entry of `n`, requested by the previous step.
 struct Walk {
     bool active = false;
-    bool first = false;
     int n = 0;
-    int idx = 0;
+    int idx = 0;  // slab index the next node goes to
     int term = 0;
-    int agent = 0;
     int slot = 0;
     uint2 ent = make_uint2(0u, 0u);
-    unsigned gS = 0u;
 };
 
-__device__ __forceinline__ void walk_step(Walk& w, int side, const DevAgents& ag, const DevSlots& slots, size_t V,
-                                          unsigned& corr_nodes) {
+__device__ __forceinline__ void walk_step(Walk& w, int side, int* slab, const DevSlots& slots, size_t V) {
     if (!w.active) return;
-    int* path = ag.path + (size_t)w.agent * ag.path_cap;
-    if (w.first) {
-        const int lenS = (int)((side == 0 ? w.ent.x : w.gS) >> 1);
-        if (side == 0) {
-            w.idx = lenS - 1;
-        } else {
-            const int len = lenS + (int)(w.ent.x >> 1) + 1;  // PathFinder.h:280-290
-            w.idx = lenS;
-            ag.path_len[w.agent] = len;
-            corr_nodes += (unsigned)len;
-        }
-        w.first = false;
-    }
     if (w.n != w.term) {
-        path[w.idx] = w.n;
+        slab[w.idx] = w.n;
         w.idx += side == 0 ? -1 : 1;
         w.n = (int)w.ent.y;
     }
     if (w.n == w.term) {
-        if (side == 1) path[w.idx] = w.term;
+        if (side == 1) slab[w.idx] = w.term;
         w.active = false;
     } else {
         w.ent = slots.vis[(size_t)w.slot * V + w.n];
     }
+}
+
+// The corridor of a found search (meet mS, mT): lane 0 of the segment takes
+// G(mS) + G(mT) + 1 ints of the slab (PathFinder.h:280-290); when they fit,
+// the agent moves (state Move, path reset) and lanes 0 and 1 start their
+// walks, else the search keeps its slot with status 4 and tries again next
+// frame.  `share` is the segment's two LDS words (base or -1, G(mS)).  Called
+// by every lane of the segment; returns true when the corridor was placed.
+__device__ __forceinline__ bool start_corridor(int l, int slot, int agent, int S, int Tg, int mS, int mT,
+                                               const uint2* vis, int* slab, const DevAgents& ag,
+                                               const DevSlots& slots, size_t V, int* share, Walk& walk,
+                                               unsigned& st_found, unsigned& st_corr) {
+    if (l < 2) {
+        // a walk still pending on these lanes (their previous search in a
+        // persistent segment) finishes first
+        while (walk.active) walk_step(walk, l, slab, slots, V);
+    }
+    if (l == 0) {
+        const int gS = (int)(vis[mS].x >> 1), gT = (int)(vis[mT].x >> 1);
+        const int len = gS + gT + 1;
+        const unsigned long long o = atomicAdd(ag.slab_top, (unsigned long long)len);
+        const bool ok = o + (unsigned long long)len <= (unsigned long long)ag.slab_cap;
+        share[0] = ok ? (int)o : -1;
+        share[1] = gS;
+        if (ok) {
+            ag.path_off[agent] = (long long)o;
+            ag.path_len[agent] = len;
+            ag.state[agent] = kMove;
+            ag.step_idx[agent] = -1;
+            ag.inflight[agent] = -1;
+            slots.agent[slot] = -1;
+            slots.status[slot] = 1;
+            ++st_found;
+            st_corr += (unsigned)len;
+        } else {
+            slots.mS[slot] = mS;
+            slots.mT[slot] = mT;
+            slots.status[slot] = 4;
+            atomicAdd(ag.slab_deferred, 1ull);
+        }
+    }
+    wave_sync();
+    const int base = share[0];
+    if (base >= 0 && l < 2) {
+        const int gS = share[1];
+        walk.active = true;
+        walk.slot = slot;
+        walk.n = l == 0 ? mS : mT;
+        walk.term = l == 0 ? S : Tg;
+        walk.idx = base + (l == 0 ? gS - 1 : gS);
+        walk.ent = vis[walk.n];
+    }
+    return base >= 0;
 }
 
 // At least 4 waves per SIMD (at most 128 VGPRs): 16,384 searches at T = 16
@@ -259,6 +295,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // holds kHashEmpty, so a compare-and-swap on it never writes
     __shared__ int hdummy[kWave];
     __shared__ int hsink[kWave];  // per-lane target of the table resets of an empty slot
+    __shared__ int wshare[RPW * 2];  // per segment: corridor base (or -1) and G(mS)
     for (int i = threadIdx.x; i < RPW * HS; i += kWave) {
         hkey[i] = kHashEmpty;
         hval[i] = kHashNone;
@@ -276,6 +313,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const size_t V = (size_t)map.V;
     const int n_active = fr.counters[1];
     const long long stride = fr.counters[9];
+    int* const slab = *ag.slab_now;  // corridor slab in use (compaction runs between launches)
 
     // per-segment search state (replicated in every lane of the segment)
     int slot = -1, agent = -1;
@@ -338,7 +376,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 pf_vn_ok = false;
                 need_pop = status == 3;
                 popped_next = false;
-                if (status == 3) {
+                has_req = true;
+                if (status == 4) {
+                    // found earlier, corridor deferred for lack of slab space
+                    start_corridor(l, slot, agent, S, Tg, slots.mS[slot], slots.mT[slot], vis, slab, ag, slots, V,
+                                   &wshare[2 * sub], walk, st_found, st_corr);
+                    has_req = false;
+                } else if (status == 3) {
                     // PathFinder.h:168-189
                     const float F0 = dist2(cS, cT) * sp.K;
                     cnt = 0;
@@ -365,7 +409,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     c_next = key_node(k0);
                     for (int i = 0; i < cnt; ++i) q[i * kWave + L] = src[1 + i];
                 }
-                has_req = true;
             }
         }
         if (__ballot(has_req) == 0ull) break;
@@ -581,7 +624,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         // one parent step of a pending corridor walk; its load was issued a
         // round ago
-        if (__ballot(walk.active) != 0ull && l < 2) walk_step(walk, l, ag, slots, V, st_corr);
+        if (__ballot(walk.active) != 0ull && l < 2) walk_step(walk, l, slab, slots, V);
         const unsigned long long nb = __ballot(has_next) & smask;
         rounds_here += has_req ? 1 : 0;
         exp_here += has_req ? __popcll(pb) : 0;
@@ -598,44 +641,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 dst[0] = m;
                 for (int i = 0; i < cnt; ++i) dst[1 + i] = q[i * kWave + L];
             }
-            if (ended && status == 1 && l < 2) {
-                // a walk still pending on these lanes (their previous search
-                // in a persistent segment) finishes first
-                while (walk.active) walk_step(walk, l, ag, slots, V, st_corr);
-                walk.active = true;
-                walk.first = true;
-                walk.agent = agent;
-                walk.slot = slot;
-                walk.n = l == 0 ? mS : mT;
-                walk.term = l == 0 ? S : Tg;
-                walk.ent = vis[walk.n];
-                if (l == 1) walk.gS = vis[mS].x;
-            }
-            if (l == 0 && (ended || park)) {
+            if (l == 0) {
                 slots.status[slot] = status;
                 st_exp += (unsigned)exp_here;
                 st_disc += (unsigned)disc_here;
                 st_rounds += (unsigned)rounds_here;
-                if (ended) {
-                    if (status == 1) {  // PathFinder.h:280-290 (the length is written by the walk)
-                        ag.state[agent] = kMove;
-                        ag.step_idx[agent] = -1;
-                        ++st_found;
-                    } else {  // PathFinder.h:268-274
-                        ag.state[agent] = kIdle;
-                        ++st_fail;
-                    }
+                if (ended && status == 2) {  // PathFinder.h:268-274
+                    ag.state[agent] = kIdle;
+                    ++st_fail;
                     slots.agent[slot] = -1;
                     ag.inflight[agent] = -1;
                 }
             }
+            if (ended && status == 1)
+                start_corridor(l, slot, agent, S, Tg, mS, mT, vis, slab, ag, slots, V, &wshare[2 * sub], walk,
+                               st_found, st_corr);
             has_req = false;
         }
     }
     // walks still pending when the last search of the workgroup ended: the
     // lanes step them together
     if (l < 2)
-        while (walk.active) walk_step(walk, l, ag, slots, V, st_corr);
+        while (walk.active) walk_step(walk, l, slab, slots, V);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         st_found += __shfl_xor(st_found, o, kWave);

@@ -50,6 +50,7 @@ struct Config {
     int RoundBudget = 256;
     float HashCell = 0;
     int Device = 0;
+    long long CorridorSlab = 0;  // corridor storage in ints; 0: sized from agents and map
 };
 
 class CuNodesMap {
@@ -93,6 +94,7 @@ class GMAPF {
         cc.round_budget = c.RoundBudget;
         cc.hash_cell = c.HashCell;
         cc.device = c.Device;
+        cc.corridor_slab = c.CorridorSlab;
         check(gmapf_init(h_, &cc));
     }
     void ManualStart() { check(gmapf_manual_start(h_)); }

@@ -54,6 +54,11 @@ typedef struct gmapf_config {
     float rot_cs, rot_sn;              /* FP32 cos/sin of the 10-degree avoidance turn;
                                           both 0: computed from 10 * pi_f / 180 */
     int device;                        /* HIP device ordinal */
+    long long corridor_slab;           /* corridor storage, ints (two buffers of this size are
+                                          kept for compaction); <= 0: min(N * V, max(N * 1024,
+                                          16 * V)); at least V.  A full slab delays a found
+                                          search's corridor to a later frame; compaction keeps
+                                          only the corridors of moving agents */
 } gmapf_config;
 
 /* Reference CuNodesMap::Node (include/gmapf/misc/CuNodesMap.h:14-21);
@@ -134,6 +139,9 @@ int gmapf_last_frame_ms(gmapf_handle h, double* out8);
 int gmapf_get_hash(gmapf_handle h, int* keys, int* order, int* n);
 /* resolved build knobs: [0] T [1] Q [2] P [3] budget [4] search grid [5] mode [6] V */
 int gmapf_get_launch_info(gmapf_handle h, int* out8);
+/* corridor slab after the last frame: [0] capacity (ints per buffer) [1] ints in use
+ * [2] compactions so far [3] corridors deferred for lack of space so far */
+int gmapf_get_slab_info(gmapf_handle h, long long* out4);
 
 #ifdef __cplusplus
 }
