@@ -399,13 +399,12 @@ struct gmapf_s {
         dfr.active = dalloc<int>(P, "active slots");
         dfr.ended = dalloc<int>(P, "ended slots");
         dfr.counters = dalloc<int>(16, "counters");
-        dfr.clear_list = dalloc<int>(P, "mark clear list");
         dfr.stats = dalloc<unsigned long long>(8, "stats");
         dfr.ncell = dmap.hnx * dmap.hny;
         dfr.hkey = dalloc<int>(Nn, "hash keys");
         dfr.hcount = dalloc<int>((size_t)dfr.ncell + 1, "hash counts");
+        HIPCHK(launch_fill(dfr.hcount, 0, dfr.ncell + 1, s1));
         dfr.hstart = dalloc<int>((size_t)dfr.ncell + 1, "hash starts");
-        dfr.hfill = dalloc<int>(dfr.ncell, "hash fill");
         dfr.hscatter = dalloc<int>(Nn, "hash scatter");
         dfr.horder = dalloc<int>(Nn, "hash order");
         dfr.hblk = dalloc<int>(dfr.ncell / scan_block + 2, "hash scan partials");

@@ -120,16 +120,14 @@ struct DevFrame {
     int* counters;   // [0] n_requests [1] n_active [2] work ticket [3] n_ended
                      // [4] n_free [5] moving count [6] n_admitted
                      // [7] blocks done in the admission count pass
-                     // [8] slots whose marks wrap this frame
+                     // [8] unused
                      // [9] ticket stride: ticket t takes active[(t * stride) % n_active]
-    int* clear_list; // P: those slots
     unsigned long long* stats;  // [0] found [1] fail [2] expansions [3] discoveries
                                 // [4] corridor nodes [5] rounds [6] admitted
     // spatial hash
     int* hkey;        // N
-    int* hcount;      // ncell + 1
+    int* hcount;      // ncell + 1, zero between frames
     int* hstart;      // ncell + 1
-    int* hfill;       // ncell
     int* hscatter;    // N
     int* horder;      // N
     int* hblk;        // scan partials over cells

@@ -296,7 +296,6 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_count(DevAgents ag, Dev
         fr.counters[5] = moving;
         fr.counters[6] = admitted;
         fr.counters[7] = 0;  // every block has counted itself: ready for the next frame
-        fr.counters[8] = 0;
         // Searches still running from earlier frames (the long ones) come
         // first in the active list and new ones last; a stride coprime with
         // the count, near 0.618 of it, deals both kinds to every workgroup,
@@ -356,22 +355,12 @@ __global__ void k_admit_slots(DevAgents ag, DevSlots sl, DevFrame fr) {
     sl.Tg[s] = ag.targ[i];
     const unsigned st = sl.stamp[s] + 1u;
     sl.stamp[s] = st;
-    if (st % 127u == 0u) fr.clear_list[atomicAdd(&fr.counters[8], 1)] = s;  // mark value wraps to 1
     sl.status[s] = 3;
     ag.inflight[i] = s;
     ag.path_len[i] = 0;  // old corridor released (PathFinder.h:104-108)
     fr.active[fr.counters[1] - admitted + k] = s;
 }
 
-// marks of the slots whose mark value wrapped: zero before their search seeds
-__global__ void k_clear_marks(DevSlots sl, DevFrame fr) {
-    const int n = fr.counters[8];
-    const long long words = sl.VP / 16;
-    for (int e = blockIdx.x; e < n; e += gridDim.x) {
-        uint4* m = reinterpret_cast<uint4*>(sl.mark + (size_t)fr.clear_list[e] * sl.VP);
-        for (long long w = threadIdx.x; w < words; w += blockDim.x) m[w] = make_uint4(0u, 0u, 0u, 0u);
-    }
-}
 
 // GMAPF.cu:247-250 for target changes made after start, read from pinned
 // host memory; the host passes at most one update per agent (the last one).
@@ -445,30 +434,27 @@ __global__ void k_slab_finish(DevAgents ag, DevFrame fr) {
 }
 
 // ------------------------------------------------------------- crowd
-// AgentsMover.h:57-70
-__global__ void k_move(DevAgents ag, DevFrame fr, CrowdParams cp) {
+// AgentsMover.h:57-70 (move), then the agent's hash cell from its moved
+// position, counted into hcount (all zero at the start of a frame: the
+// scatter counts every cell back down)
+__global__ void k_move_key(DevMap map, DevAgents ag, DevFrame fr) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ag.n || fr.counters[5] == 0) return;
-    if (!ag.in_m[i] || ag.step_idx[i] == -1) return;
-    const float step = *fr.stepv;
     float2 p = ag.pos[i];
-    const float2 sp = ag.step_pos[i];
-    float dx = sp.x - p.x, dy = sp.y - p.y;
-    float len = sqrtf(dx * dx + dy * dy);
-    if (len <= step) {
-        p = sp;
-    } else {
-        float ux = dx / len, uy = dy / len;
-        p.x = p.x + ux * step;
-        p.y = p.y + uy * step;
+    if (ag.in_m[i] && ag.step_idx[i] != -1) {
+        const float step = *fr.stepv;
+        const float2 sp = ag.step_pos[i];
+        float dx = sp.x - p.x, dy = sp.y - p.y;
+        float len = sqrtf(dx * dx + dy * dy);
+        if (len <= step) {
+            p = sp;
+        } else {
+            float ux = dx / len, uy = dy / len;
+            p.x = p.x + ux * step;
+            p.y = p.y + uy * step;
+        }
+        ag.pos[i] = p;
     }
-    ag.pos[i] = p;
-}
-
-__global__ void k_hash_key(DevMap map, DevAgents ag, DevFrame fr) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ag.n || fr.counters[5] == 0) return;
-    const float2 p = ag.pos[i];
     int key = -1;
     if (!(isnan(p.x) || isnan(p.y))) {
         int cx = cell_of(p.x, map.hx0, map.hinv, map.hnx);
@@ -484,7 +470,8 @@ __global__ void k_hash_scatter(DevAgents ag, DevFrame fr) {
     if (i >= ag.n || fr.counters[5] == 0) return;
     const int key = fr.hkey[i];
     if (key < 0) return;
-    int p = fr.hstart[key] + atomicAdd(&fr.hfill[key], 1);
+    // fill the cell from its end down; the count returns to zero for the next frame
+    int p = fr.hstart[key] + atomicSub(&fr.hcount[key], 1) - 1;
     fr.hscatter[p] = i;
 }
 
@@ -618,7 +605,6 @@ hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& 
     hipLaunchKernelGGL(k_admit_count, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
     hipLaunchKernelGGL(k_admit_scatter, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
     hipLaunchKernelGGL(k_admit_slots, dim3(nblk(sl.P, 256)), dim3(256), 0, st, ag, sl, fr);
-    hipLaunchKernelGGL(k_clear_marks, dim3(256), dim3(256), 0, st, sl, fr);
     return hipGetLastError();
 }
 
@@ -641,10 +627,7 @@ hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& 
                         const CrowdParams& cp, hipStream_t st) {
     hipError_t e;
     const int nb = nblk(ag.n, 256);
-    hipLaunchKernelGGL(k_move, dim3(nb), dim3(256), 0, st, ag, fr, cp);
-    if ((e = launch_fill(fr.hcount, 0, fr.ncell + 1, st)) != hipSuccess) return e;
-    if ((e = launch_fill(fr.hfill, 0, fr.ncell, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_hash_key, dim3(nb), dim3(256), 0, st, map, ag, fr);
+    hipLaunchKernelGGL(k_move_key, dim3(nb), dim3(256), 0, st, map, ag, fr);
     if ((e = exscan(fr.hcount, fr.ncell, fr.hblk, fr.hstart, &fr.hstart[fr.ncell], st)) != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_hash_scatter, dim3(nb), dim3(256), 0, st, ag, fr);

@@ -383,6 +383,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                    &wshare[2 * sub], walk, st_found, st_corr);
                     has_req = false;
                 } else if (status == 3) {
+                    if (mval == 1u) {
+                        // generation wrapped (every 127 searches of the slot):
+                        // zero the slot's marks before the seeds
+                        uint4* m4 = reinterpret_cast<uint4*>(mark);
+                        for (long long w = l; w < slots.VP / 16; w += T) m4[w] = make_uint4(0u, 0u, 0u, 0u);
+                        wave_sync();
+                    }
                     // PathFinder.h:168-189
                     const float F0 = dist2(cS, cT) * sp.K;
                     cnt = 0;
