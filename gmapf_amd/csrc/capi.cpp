@@ -59,7 +59,9 @@ struct gmapf_s {
     std::vector<int> state, curr, targ, gen;
     std::vector<float> targ_pos;
     // target changes made after start, applied at the next AsyncStep
-    std::vector<int> upd_ids, upd_targ;
+    // at most one pending update per agent: upd_pos[id] is its index in
+    // upd_ids, or -1; a later SetAgentTargPos overwrites the entry in place
+    std::vector<int> upd_ids, upd_targ, upd_pos;
     std::vector<float> upd_xy;
 
     // device
@@ -193,8 +195,8 @@ struct gmapf_s {
         if (started) throw ApiError(GMAPF_E_STATE, "already started");
         if (!map_filled) throw ApiError(GMAPF_E_STATE, "map is not filled");
         check_config();
-        // the search packs node ids into 23-bit fields of its claim hash
-        if (nav.V >= (1 << 23)) throw ApiError(GMAPF_E_ARG, "navmesh has more than 2^23 - 1 nodes");
+        // node ids, corridor offsets and slab indices are 32-bit
+        if (nav.V >= (1 << 30)) throw ApiError(GMAPF_E_ARG, "navmesh has more than 2^30 - 1 nodes");
         dev = cfg.device;
         HIPCHK(hipSetDevice(dev));
         started = true;  // from here release() frees what was allocated
@@ -506,14 +508,12 @@ struct gmapf_s {
         *h_step = cfg.agent_speed * dt;  // AgentsMover.h:299
         h_upd[0] = 0;
         if (!upd_ids.empty()) {
-            compact_updates();  // one update per agent: the last one wins
-            const int n = (int)upd_ids.size();
-            for (int k = 0; k < n; k++) {
-                h_upd[1 + k] = upd_ids[k];
-                h_upd[1 + upd_cap + k] = upd_targ[k];
-                h_upd_xy[k] = make_float2(upd_xy[2 * k], upd_xy[2 * k + 1]);
-            }
+            const int n = (int)upd_ids.size();  // one per agent (the last one made)
+            std::memcpy(h_upd + 1, upd_ids.data(), (size_t)n * sizeof(int));
+            std::memcpy(h_upd + 1 + upd_cap, upd_targ.data(), (size_t)n * sizeof(int));
+            std::memcpy(h_upd_xy, upd_xy.data(), (size_t)n * sizeof(float2));
             h_upd[0] = n;
+            for (int id : upd_ids) upd_pos[id] = -1;
             upd_ids.clear();
             upd_targ.clear();
             upd_xy.clear();
@@ -679,30 +679,20 @@ struct gmapf_s {
             targ_pos[2 * id + 1] = y;
             gen[id]++;
         } else {
-            upd_ids.push_back(id);
-            upd_targ.push_back(node);
-            upd_xy.push_back(x);
-            upd_xy.push_back(y);
-            if ((int)upd_ids.size() > upd_cap) compact_updates();
+            if ((int)upd_pos.size() < n_agents) upd_pos.assign(n_agents, -1);
+            int& k = upd_pos[id];
+            if (k >= 0) {  // an update of this agent is pending: the last one wins
+                upd_targ[k] = node;
+                upd_xy[2 * (size_t)k] = x;
+                upd_xy[2 * (size_t)k + 1] = y;
+            } else {
+                k = (int)upd_ids.size();
+                upd_ids.push_back(id);
+                upd_targ.push_back(node);
+                upd_xy.push_back(x);
+                upd_xy.push_back(y);
+            }
         }
-    }
-
-    // keep only the last update per agent (they would overwrite each other)
-    void compact_updates() {
-        std::vector<int> last(n_agents, -1);
-        for (int k = 0; k < (int)upd_ids.size(); k++) last[upd_ids[k]] = k;
-        std::vector<int> ids, tg;
-        std::vector<float> xy;
-        for (int k = 0; k < (int)upd_ids.size(); k++) {
-            if (last[upd_ids[k]] != k) continue;
-            ids.push_back(upd_ids[k]);
-            tg.push_back(upd_targ[k]);
-            xy.push_back(upd_xy[2 * k]);
-            xy.push_back(upd_xy[2 * k + 1]);
-        }
-        upd_ids.swap(ids);
-        upd_targ.swap(tg);
-        upd_xy.swap(xy);
     }
 
     void profiler_dump() const {
