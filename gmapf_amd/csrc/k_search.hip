@@ -278,10 +278,18 @@ __device__ __forceinline__ bool start_corridor(int l, int slot, int agent, int S
     return base >= 0;
 }
 
-// At least 4 waves per SIMD (at most 128 VGPRs): 16,384 searches at T = 16
-// need 4 resident waves per SIMD to run one search per segment.
+// Resident waves per SIMD the launch needs to run one search per segment at
+// 16,384 searches: T/4 (4 at T = 16: at most 128 VGPRs; 2 at T <= 8: at most
+// 256).
 template <int T>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_search(DevMap map, DevAgents ag, DevSlots slots, DevFrame fr,
+constexpr int search_waves_per_eu() {
+    return T >= 16 ? 4 : 2;
+}
+
+// QC > 0 is the queue capacity fixed at compile time (the heap loops unroll
+// with constant bounds); QC = 0 takes it from the launch.
+template <int T, int QC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves_per_eu<T>()))) void k_search(DevMap map, DevAgents ag, DevSlots slots, DevFrame fr,
                                                SearchParams sp, int persistent) {
     extern __shared__ unsigned long long q[];  // 64 * (Q + 4) keys: Q heap rows + the spare rows
     constexpr int RPW = kWave / T;
@@ -308,7 +316,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int segbase = sub * 3 * T;
     const unsigned long long smask = seg_mask<T>(sub);
     const unsigned long long below = smask & ((1ull << L) - 1ull);
-    const int Q = sp.Q;
+    const int Q = QC > 0 ? QC : sp.Q;
     const int depth = heap_depth(Q);
     const size_t V = (size_t)map.V;
     const int n_active = fr.counters[1];
@@ -609,6 +617,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             cnt -= shrink ? 1 : 0;
             const unsigned long long last = q[(shrink ? cnt : Q) * kWave + L];
             heap_sift_down(q, L, Q, depth, repl || (shrink && cnt > 0), cnt, repl ? d0 : last);
+            // marks of the next node's neighbours, as soon as its record is
+            // in (it was requested before the sift): every write of this
+            // round is visible (barrier above) and none can happen before the
+            // next round reads them
+            if (has_next) {
+                const int a0 = pf_r0.x, a1 = a0 >= 0 ? pf_r0.y : -1, a2 = a1 >= 0 ? pf_r0.z : -1;
+                pf_vn[0] = mark[a0 >= 0 ? a0 : 0];
+                pf_vn[1] = mark[a1 >= 0 ? a1 : 0];
+                pf_vn[2] = mark[a2 >= 0 ? a2 : 0];
+                pf_vn_ok = true;
+            }
             // the items still to push, at most two: the dealt items other
             // than m (and d0 when it replaced the root)
             const bool skip0 = repl || m == d0;
@@ -619,16 +638,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         popped_next = has_next;
         c_next = pf_c;
-        // visited entries of the next node's neighbours: every write of this
-        // round is visible (barrier above) and none can happen before the
-        // next round reads them
-        if (has_next) {
-            const int a0 = pf_r0.x, a1 = a0 >= 0 ? pf_r0.y : -1, a2 = a1 >= 0 ? pf_r0.z : -1;
-            pf_vn[0] = mark[a0 >= 0 ? a0 : 0];
-            pf_vn[1] = mark[a1 >= 0 ? a1 : 0];
-            pf_vn[2] = mark[a2 >= 0 ? a2 : 0];
-            pf_vn_ok = true;
-        }
         // one parent step of a pending corridor walk; its load was issued a
         // round ago
         if (__ballot(walk.active) != 0ull && l < 2) walk_step(walk, l, slab, slots, V);
@@ -689,14 +698,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
 }
 
-const void* search_fn(int T) {
+// Compile-time queue capacities (the tuned ones) for T <= 8, where the
+// unrolled heap fits in registers at the occupancy the launch needs; other
+// (T, Q) use the generic form.
+template <int T>
+constexpr bool search_q_fixed(int Q) {
+    return T <= 8 && (Q == 16 || Q == 32);
+}
+
+template <int T>
+const void* search_fn_q(int Q) {
+    if constexpr (T <= 8) {
+        if (search_q_fixed<T>(Q)) return Q == 16 ? (const void*)k_search<T, 16> : (const void*)k_search<T, 32>;
+    }
+    return (const void*)k_search<T, 0>;
+}
+
+const void* search_fn(int T, int Q) {
     switch (T) {
-        case 2: return (const void*)k_search<2>;
-        case 4: return (const void*)k_search<4>;
-        case 8: return (const void*)k_search<8>;
-        case 16: return (const void*)k_search<16>;
-        case 32: return (const void*)k_search<32>;
-        case 64: return (const void*)k_search<64>;
+        case 2: return search_fn_q<2>(Q);
+        case 4: return search_fn_q<4>(Q);
+        case 8: return search_fn_q<8>(Q);
+        case 16: return search_fn_q<16>(Q);
+        case 32: return search_fn_q<32>(Q);
+        case 64: return search_fn_q<64>(Q);
         default: return nullptr;
     }
 }
@@ -706,7 +731,7 @@ const void* search_fn(int T) {
 int search_lds_bytes(int Q) { return kWave * (Q + kSpareRows) * (int)sizeof(unsigned long long); }
 
 hipError_t search_prepare(int T, int Q) {
-    const void* fn = search_fn(T);
+    const void* fn = search_fn(T, Q);
     if (!fn) return hipErrorInvalidValue;
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, search_lds_bytes(Q));
 }
@@ -714,28 +739,41 @@ hipError_t search_prepare(int T, int Q) {
 hipError_t launch_search(const DevMap& map, const DevAgents& ag, const DevSlots& slots, const DevFrame& fr,
                          const SearchParams& sp, int grid, int persistent, hipStream_t st) {
     const size_t lds = (size_t)search_lds_bytes(sp.Q);
-#define GMAPF_SEARCH_CASE(TT)                                                                        \
-    case TT:                                                                                         \
-        hipLaunchKernelGGL(k_search<TT>, dim3(grid), dim3(kWave), lds, st, map, ag, slots, fr, sp,   \
-                           persistent);                                                              \
+#define GMAPF_SEARCH_LAUNCH(TT, QQ)                                                                  \
+    hipLaunchKernelGGL((k_search<TT, QQ>), dim3(grid), dim3(kWave), lds, st, map, ag, slots, fr, sp, \
+                       persistent)
+#define GMAPF_SEARCH_CASE(TT)                          \
+    case TT:                                           \
+        if (!search_q_fixed<TT>(sp.Q))                 \
+            GMAPF_SEARCH_LAUNCH(TT, 0);                \
+        else if (sp.Q == 16)                           \
+            GMAPF_SEARCH_LAUNCH(TT, 16);               \
+        else                                           \
+            GMAPF_SEARCH_LAUNCH(TT, 32);               \
+        return hipGetLastError();
+#define GMAPF_SEARCH_CASE_GENERIC(TT)                  \
+    case TT:                                           \
+        GMAPF_SEARCH_LAUNCH(TT, 0);                    \
         return hipGetLastError();
     switch (sp.T) {
         GMAPF_SEARCH_CASE(2)
         GMAPF_SEARCH_CASE(4)
         GMAPF_SEARCH_CASE(8)
-        GMAPF_SEARCH_CASE(16)
-        GMAPF_SEARCH_CASE(32)
-        GMAPF_SEARCH_CASE(64)
+        GMAPF_SEARCH_CASE_GENERIC(16)
+        GMAPF_SEARCH_CASE_GENERIC(32)
+        GMAPF_SEARCH_CASE_GENERIC(64)
         default:
             return hipErrorInvalidValue;
     }
+#undef GMAPF_SEARCH_CASE_GENERIC
 #undef GMAPF_SEARCH_CASE
+#undef GMAPF_SEARCH_LAUNCH
 }
 
 hipError_t search_occupancy(int T, int Q, int* blocks_per_cu) {
     hipError_t e = search_prepare(T, Q);
     if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, search_fn(T), kWave,
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, search_fn(T, Q), kWave,
                                                         (size_t)search_lds_bytes(Q));
 }
 
