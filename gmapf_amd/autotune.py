@@ -42,8 +42,8 @@ BUDGETS = (64, 96, 128, 192, 256, 384, 512)
 
 
 def lds_fits(Q):
-    # Q heap rows + four spare rows of 64 keys, plus about 10 KB of static tables
-    return 64 * (Q + 4) * 8 + 10 * 1024 <= LDS_BYTES
+    # Q heap rows + five spare rows of 64 keys, plus about 10 KB of static tables
+    return 64 * (Q + 5) * 8 + 10 * 1024 <= LDS_BYTES
 
 
 def _confirm(sc, rows, warmup, steps, log):

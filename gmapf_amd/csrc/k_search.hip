@@ -67,14 +67,16 @@ __device__ __forceinline__ float dist2(float2 a, float2 b) {
 
 // ---- per-lane 4-ary min-heap in LDS, "keep the Q smallest" (CuQueue.h:33-70).
 // Entry i of lane L is q[i * 64 + L]; node i's children are 4i+1 .. 4i+4
-// (a 16-item heap is two levels deep).  Rows Q .. Q+3 are the lane's spare
-// entries: the target of the writes of a lane that has nothing to move and
-// the padding of a child group that runs past the heap, so the sifts run a
-// fixed number of levels without branches and the wave leaves a sift as soon
-// as no lane moves (a ballot per level).  All lanes of the wave call these
-// functions; `on` selects the lanes that act.
+// (a 16-item heap is two levels deep).  Every entry at or beyond the lane's
+// count holds kKeyNone, and so do rows Q .. Q+3 (the padding of a child group
+// that runs past the heap), so a sift reads four children with no bounds
+// test.  Row Q+4 is the target of the writes of a lane that has nothing to
+// move.  The sifts run a fixed number of levels without branches and the wave
+// leaves a sift as soon as no lane moves (a ballot per level).  All lanes of
+// the wave call these functions; `on` selects the lanes that act.
 constexpr unsigned long long kKeyNone = ~0ull;  // larger than every key
-constexpr int kSpareRows = 4;
+constexpr int kPadRows = 4;
+constexpr int kSpareRows = kPadRows + 1;
 
 // levels below the root of a Q-item 4-ary heap
 __device__ __forceinline__ int heap_depth(int Q) {
@@ -93,39 +95,36 @@ __device__ __forceinline__ void heap_sift_up(unsigned long long* q, int L, int Q
         const int p = (i - 1) >> 2;
         const unsigned long long pv = q[(hp ? p : Q) * kWave + L];
         const bool move = !done && hp && pv > k;
-        q[(move ? i : Q) * kWave + L] = pv;
+        q[(move ? i : Q + kPadRows) * kWave + L] = pv;
         i = move ? p : i;
         done = !move;
     }
-    q[(on ? i : Q) * kWave + L] = k;
+    q[(on ? i : Q + kPadRows) * kWave + L] = k;
 }
 
-// Sift key k down from a hole at the root of a heap of n items.
-__device__ __forceinline__ void heap_sift_down(unsigned long long* q, int L, int Q, int depth, bool on, int n,
+// Sift key k down from a hole at the root (the entries past the heap's items
+// hold kKeyNone).
+__device__ __forceinline__ void heap_sift_down(unsigned long long* q, int L, int Q, int depth, bool on,
                                                unsigned long long k) {
     int i = 0;
     bool done = !on;
     for (int d = 0; d < depth; ++d) {
         if (__ballot(!done) == 0ull) break;
         const int c0 = 4 * i + 1;
-        const int cb = c0 < Q ? c0 : Q;  // rows cb .. cb+3 lie within the heap and spare rows
+        const int cb = c0 < Q ? c0 : Q;  // rows cb .. cb+3 lie within the heap and padding rows
         const unsigned long long* r = q + cb * kWave + L;
-        unsigned long long v0 = r[0], v1 = r[kWave], v2 = r[2 * kWave], v3 = r[3 * kWave];
-        v0 = c0 < n ? v0 : kKeyNone;
-        v1 = c0 + 1 < n ? v1 : kKeyNone;
-        v2 = c0 + 2 < n ? v2 : kKeyNone;
-        v3 = c0 + 3 < n ? v3 : kKeyNone;
+        const unsigned long long v0 = r[0], v1 = r[kWave], v2 = r[2 * kWave], v3 = r[3 * kWave];
         const bool b1 = v1 < v0, b3 = v3 < v2;
         const unsigned long long m01 = b1 ? v1 : v0, m23 = b3 ? v3 : v2;
         const bool b23 = m23 < m01;
         const unsigned long long cv = b23 ? m23 : m01;
         const int ci = c0 + (b23 ? (b3 ? 3 : 2) : (b1 ? 1 : 0));
         const bool move = !done && cv < k;
-        q[(move ? i : Q) * kWave + L] = cv;
+        q[(move ? i : Q + kPadRows) * kWave + L] = cv;
         i = move ? ci : i;
         done = !move;
     }
-    q[(on ? i : Q) * kWave + L] = k;
+    q[(on ? i : Q + kPadRows) * kWave + L] = k;
 }
 
 // Push key k (lanes with k != kKeyNone) into a heap bounded to `cap` items:
@@ -318,6 +317,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     const unsigned long long below = smask & ((1ull << L) - 1ull);
     const int Q = QC > 0 ? QC : sp.Q;
     const int depth = heap_depth(Q);
+    for (int r = 0; r < kPadRows; ++r) q[(Q + r) * kWave + threadIdx.x] = kKeyNone;
     const size_t V = (size_t)map.V;
     const int n_active = fr.counters[1];
     const long long stride = fr.counters[9];
@@ -401,6 +401,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                     // PathFinder.h:168-189
                     const float F0 = dist2(cS, cT) * sp.K;
                     cnt = 0;
+                    for (int i = 0; i < Q; ++i) q[i * kWave + L] = kKeyNone;
                     if (l == 0) {
                         q[L] = make_key(F0, S);
                         cnt = 1;
@@ -422,7 +423,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                     const unsigned long long k0 = src[0];
                     popped_next = k0 != kKeyNone;
                     c_next = key_node(k0);
-                    for (int i = 0; i < cnt; ++i) q[i * kWave + L] = src[1 + i];
+                    for (int i = 0; i < Q; ++i) q[i * kWave + L] = i < cnt ? src[1 + i] : kKeyNone;
                 }
             }
         }
@@ -436,8 +437,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                 c_next = key_node(q[L]);
                 --cnt;
             }
-            const unsigned long long last = q[(on ? cnt : Q) * kWave + L];
-            heap_sift_down(q, L, Q, depth, on && cnt > 0, cnt, last);
+            const int lrow = on ? cnt : Q + kPadRows;
+            const unsigned long long last = q[lrow * kWave + L];
+            q[lrow * kWave + L] = kKeyNone;  // the vacated entry
+            heap_sift_down(q, L, Q, depth, on && cnt > 0, last);
             if (has_req && need_pop) popped_next = on;
             need_pop = false;
         }
@@ -596,7 +599,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         d0 = cont && l < tot ? d0 : kKeyNone;
         d1 = cont && l + T < tot ? d1 : kKeyNone;
         d2 = cont && l + 2 * T < tot ? d2 : kKeyNone;
-        const unsigned long long top = cont && cnt > 0 ? q[L] : kKeyNone;
+        const unsigned long long top = cont ? q[L] : kKeyNone;  // kKeyNone when empty
         unsigned long long m = top < d0 ? top : d0;
         m = d1 < m ? d1 : m;
         m = d2 < m ? d2 : m;
@@ -615,8 +618,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             const bool repl = from_top && d0 != kKeyNone;
             const bool shrink = from_top && !repl;
             cnt -= shrink ? 1 : 0;
-            const unsigned long long last = q[(shrink ? cnt : Q) * kWave + L];
-            heap_sift_down(q, L, Q, depth, repl || (shrink && cnt > 0), cnt, repl ? d0 : last);
+            const int lrow = shrink ? cnt : Q + kPadRows;
+            const unsigned long long last = q[lrow * kWave + L];
+            q[lrow * kWave + L] = kKeyNone;  // the vacated entry
+            heap_sift_down(q, L, Q, depth, repl || (shrink && cnt > 0), repl ? d0 : last);
             // marks of the next node's neighbours, as soon as its record is
             // in (it was requested before the sift): every write of this
             // round is visible (barrier above) and none can happen before the
