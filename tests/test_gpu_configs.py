@@ -108,3 +108,64 @@ def test_config4_full_deterministic():
     assert np.array_equal(outs[0][1], outs[1][1])
     assert not np.isnan(outs[0][0]).any()
     assert outs[0][2] == outs[1][2]
+
+
+def test_config3_maze_full_corridors_are_tree_paths(oracle):
+    """The maze workload at full size (4096^2 spanning-tree maze, 16,384
+    agents, log-uniform 10-4000 targets, continuous re-path) in the batched
+    form at the tuned point.  The free space is simply connected, so the
+    triangle adjacency graph is a tree and the only simple path between two
+    nodes is the corridor PathFinder.h:276-306 describes: every corridor the
+    GPU produces must start next to its start node, step between adjacent
+    nodes, never repeat a node and end on the target node.  A sample of the
+    searches is also compared with the oracle (found flag and corridor)."""
+    from gmapf_amd.runner import make_world
+    make, keep = locator()
+    sc = S.config3(make, n_agents=16384, cells=64)
+    T, Q, B = 4, 16, 256
+    g = make_world(sc, G.MODE_BATCHED, T, Q, B)
+    tri, nbr, _ = g.map_arrays()
+    V = len(tri)
+    assert (nbr >= 0).sum() == 2 * (V - 1) and len(np.unique(S.components(nbr))) == 1  # a tree
+    om = oracle.OracleMap(tri, nbr)
+    targets = np.array(sc.targets, copy=True)
+    counters = np.array(sc.counters, copy=True)
+    _, state, nodes = g.agents(with_nodes=True)
+    checked = compared = moved = 0
+    rng = np.random.default_rng(3)
+    for f in range(12):
+        prev_state, prev_nodes = state, nodes
+        tgt_nodes = g.locate(targets)
+        print(f"maze full frame {f}", flush=True)
+        g.step(sc.dt)
+        xy, state, nodes = g.agents(with_nodes=True)
+        found = np.nonzero((prev_state == G.SEARCH) & (state == G.MOVE))[0]
+        failed = np.nonzero((prev_state == G.SEARCH) & (state == G.IDLE))[0]
+        moved += len(found)
+        for i in rng.choice(found, size=min(len(found), 1200), replace=False):
+            c = g.corridor(int(i))
+            s_node, t_node = int(prev_nodes[i]), int(tgt_nodes[i])
+            assert c is not None and len(c) >= 1 and c[-1] == t_node, (f, i)
+            walk = np.concatenate([[s_node], c])
+            assert (nbr[walk[:-1]] == walk[1:, None]).any(axis=1).all(), (f, i)
+            assert len(np.unique(walk)) == len(walk), (f, i)
+            checked += 1
+        if f < 3:
+            for i in list(rng.choice(found, size=min(len(found), 120), replace=False)) + \
+                    list(rng.choice(failed, size=min(len(failed), 40), replace=False)):
+                s_node, t_node = int(prev_nodes[i]), int(tgt_nodes[i])
+                ok, oc, _ = om.search(s_node, t_node, T, Q)
+                assert ok == (state[i] == G.MOVE), (f, i)
+                if ok:
+                    assert np.array_equal(g.corridor(int(i)), oc), (f, i)
+                compared += 1
+        done = np.nonzero(state != G.SEARCH)[0]
+        if len(done):
+            tg = S.log_uniform_targets(sc.seed, 302, xy[done], 10.0, 4000.0, sc.width, sc.height, sc.locate,
+                                       counters, ids=done)
+            targets[done] = tg
+            g.set_agent_targets(done.astype(np.int32), tg)
+    g.close()
+    print(f"maze full: {moved} corridors, {checked} checked as tree paths, {compared} compared with the oracle",
+          flush=True)
+    assert moved > 20000 and checked > 10000 and compared > 300
