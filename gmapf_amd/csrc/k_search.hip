@@ -22,7 +22,7 @@
 // oracle's serial restatement.
 //
 // Mapping: one workgroup = one wave64 = 64/T searches ("segments").  Each
-// lane's queue is a binary min-heap of 64-bit keys (F bits made monotone,
+// lane's queue is a 4-ary min-heap of 64-bit keys (F bits made monotone,
 // node id) in LDS, entry i of lane L at q[i*64+L] (bank-conflict free);
 // when full, a smaller key replaces the largest leaf and sifts up.  Claim
 // prefix sums and the first-meet choice are wave ballots + popcounts;
