@@ -61,6 +61,7 @@ class _Config(C.Structure):
         ("rot_sn", C.c_float),
         ("device", C.c_int),
         ("corridor_slab", C.c_longlong),
+        ("searches_per_wave", C.c_int),
     ]
 
 
@@ -93,6 +94,7 @@ class Config:
     rot_sn: float = 0.0
     device: int = 0
     corridor_slab: int = 0
+    searches_per_wave: int = 0
 
     def _to_c(self) -> _Config:
         c = _Config()

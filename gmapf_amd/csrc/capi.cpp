@@ -188,6 +188,8 @@ struct gmapf_s {
             throw ApiError(GMAPF_E_ARG, "unknown step_mode");
         if (cfg.step_mode == GMAPF_MODE_BATCHED && cfg.round_budget < 1)
             throw ApiError(GMAPF_E_ARG, "batched mode needs round_budget >= 1");
+        if (cfg.searches_per_wave > 64 / T)
+            throw ApiError(GMAPF_E_ARG, "searches_per_wave must be at most 64 / PathFinderThreadsPerAgents");
     }
 
     void manual_start() {
@@ -440,9 +442,9 @@ struct gmapf_s {
         dfr.h_slab_out = h_slab;
 
         // ---- search launch shape
-        sp = SearchParams{T, Q, cfg.path_finder_heuristic_k, 0};
+        const int rpw = cfg.searches_per_wave > 0 ? cfg.searches_per_wave : 64 / T;
+        sp = SearchParams{T, Q, cfg.path_finder_heuristic_k, 0, rpw};
         HIPCHK(search_prepare(T, Q));
-        const int rpw = 64 / T;
         if (cfg.step_mode == GMAPF_MODE_BATCHED) {
             int per_cu = 0, cus = 0;
             HIPCHK(search_occupancy(T, Q, &per_cu));

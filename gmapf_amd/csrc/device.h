@@ -154,6 +154,7 @@ struct SearchParams {
     int Q;
     float K;
     int budget;  // max rounds per search per launch; <= 0: unbounded
+    int seg;     // searches per workgroup (segments 0 .. seg-1 take work), 1 .. 64 / T
 };
 
 struct CrowdParams {

@@ -209,7 +209,14 @@ struct Walk {
     uint2 ent = make_uint2(0u, 0u);
 };
 
-__device__ __forceinline__ void walk_step(Walk& w, int side, int* slab, const DevSlots& slots, size_t V) {
+// The slab pointer is read from memory, so the compiler cannot tell its
+// address space; stores through a generic pointer become flat stores, which
+// count on the LDS counter as well and so make LDS waits wait for them.
+typedef __attribute__((address_space(1))) int global_int;
+
+// A step in two halves: `walk_advance` writes the node whose entry arrived
+// and moves to its parent; `walk_request` asks for the parent's entry.
+__device__ __forceinline__ void walk_advance(Walk& w, int side, global_int* slab) {
     if (!w.active) return;
     if (w.n != w.term) {
         slab[w.idx] = w.n;
@@ -219,9 +226,16 @@ __device__ __forceinline__ void walk_step(Walk& w, int side, int* slab, const De
     if (w.n == w.term) {
         if (side == 1) slab[w.idx] = w.term;
         w.active = false;
-    } else {
-        w.ent = slots.vis[(size_t)w.slot * V + w.n];
     }
+}
+
+__device__ __forceinline__ void walk_request(Walk& w, const DevSlots& slots, size_t V) {
+    if (w.active) w.ent = slots.vis[(size_t)w.slot * V + w.n];
+}
+
+__device__ __forceinline__ void walk_step(Walk& w, int side, global_int* slab, const DevSlots& slots, size_t V) {
+    walk_advance(w, side, slab);
+    walk_request(w, slots, V);
 }
 
 // The corridor of a found search (meet mS, mT): lane 0 of the segment takes
@@ -231,7 +245,7 @@ __device__ __forceinline__ void walk_step(Walk& w, int side, int* slab, const De
 // frame.  `share` is the segment's two LDS words (base or -1, G(mS)).  Called
 // by every lane of the segment; returns true when the corridor was placed.
 __device__ __forceinline__ bool start_corridor(int l, int slot, int agent, int S, int Tg, int mS, int mT,
-                                               const uint2* vis, int* slab, const DevAgents& ag,
+                                               const uint2* vis, global_int* slab, const DevAgents& ag,
                                                const DevSlots& slots, size_t V, int* share, Walk& walk,
                                                unsigned& st_found, unsigned& st_corr) {
     if (l < 2) {
@@ -320,12 +334,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     for (int r = 0; r < kPadRows; ++r) q[(Q + r) * kWave + threadIdx.x] = kKeyNone;
     const size_t V = (size_t)map.V;
     const int n_active = fr.counters[1];
+    const bool one_each = (long long)n_active <= (long long)gridDim.x * sp.seg;
     const long long stride = fr.counters[9];
-    int* const slab = *ag.slab_now;  // corridor slab in use (compaction runs between launches)
+    // corridor slab in use (compaction runs between launches)
+    global_int* const slab = (global_int*)*ag.slab_now;
 
     // per-segment search state (replicated in every lane of the segment)
     int slot = -1, agent = -1;
-    bool has_req = false, no_more = false, took_plain = false;
+    bool has_req = false, no_more = sub >= sp.seg, took_plain = false;
     // the node this lane expands in the coming round: taken from the heap
     // and the items dealt to the lane at the end of the previous round, or
     // popped at the top of the first round of a fresh search
@@ -357,11 +373,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         // them)
         if (__ballot(!has_req && !no_more) != 0ull && !has_req && !no_more) {
             int item = -1;
-            if (persistent) {
+            if (persistent && !one_each) {
                 if (l == 0) item = atomicAdd(&fr.counters[2], 1);
                 item = __shfl(item, sub * T, kWave);
             } else if (!took_plain) {
-                item = blockIdx.x * RPW + sub;
+                // every search has a segment of its own: segment i takes
+                // item i, once (the plain form always does)
+                item = blockIdx.x * sp.seg + sub;
                 took_plain = true;
             }
             if (item < 0 || item >= n_active) {
@@ -417,13 +435,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                     status = 0;
                 } else {
                     // parked: [node to expand next or kKeyNone, heap items]
+                    // (at most Q - 1 heap items).  Loads go in groups of 8
+                    // with clamped in-bounds indices and the count applied
+                    // afterwards, so a group waits for memory once
                     const size_t qb = ((size_t)slot * T + l);
                     cnt = slots.qcnt[qb];
                     const unsigned long long* src = slots.qsave + qb * (size_t)Q;
                     const unsigned long long k0 = src[0];
                     popped_next = k0 != kKeyNone;
                     c_next = key_node(k0);
-                    for (int i = 0; i < Q; ++i) q[i * kWave + L] = i < cnt ? src[1 + i] : kKeyNone;
+                    for (int i0 = 0; i0 < Q - 1; i0 += 8) {
+                        unsigned long long x[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) x[j] = src[1 + min(i0 + j, Q - 2)];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            if (i0 + j < Q - 1) q[(i0 + j) * kWave + L] = i0 + j < cnt ? x[j] : kKeyNone;
+                    }
+                    q[(Q - 1) * kWave + L] = kKeyNone;
                 }
             }
         }
@@ -486,6 +515,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             prior[s] = nn[s] >= 0 && (mk[s] >> 1) == mval;
             pdir[s] = (int)(mk[s] & 1u);
         }
+        // one parent step of a pending corridor walk: the entry was
+        // requested at the end of the last round, right after the marks just
+        // used, so it has arrived with them (the vector memory counter is in
+        // order: a wait for a younger load waits for every older one, so the
+        // request comes after the round's other loads, never before them)
+        if (__ballot(walk.active) != 0ull && l < 2) walk_advance(walk, l, slab);
 
         // ---- this round's candidates on unmarked nodes: place each node in
         // the segment's table (linear probing, first compare-and-swap wins
@@ -643,9 +678,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         }
         popped_next = has_next;
         c_next = pf_c;
-        // one parent step of a pending corridor walk; its load was issued a
-        // round ago
-        if (__ballot(walk.active) != 0ull && l < 2) walk_step(walk, l, slab, slots, V);
+        if (l < 2) walk_request(walk, slots, V);
         const unsigned long long nb = __ballot(has_next) & smask;
         rounds_here += has_req ? 1 : 0;
         exp_here += has_req ? __popcll(pb) : 0;

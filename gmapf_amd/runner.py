@@ -29,11 +29,12 @@ class Locators:
         return lambda p: g.locate(p)
 
 
-def make_world(sc, mode, T, Q, budget, profile=False, hash_cell=0.0):
+def make_world(sc, mode, T, Q, budget, profile=False, hash_cell=0.0, spw=0):
     cfg = Config(agents_max_count=sc.n, agent_radius=sc.radius, agent_speed=sc.speed,
                  path_finder_parallel_agents=sc.P, path_finder_threads_per_agents=T,
                  path_finder_queue_capacity=Q, path_finder_heuristic_k=sc.K, step_mode=mode,
-                 round_budget=max(1, budget), is_profiler=profile, hash_cell=hash_cell)
+                 round_budget=max(1, budget), is_profiler=profile, hash_cell=hash_cell,
+                 searches_per_wave=spw)
     g = GMAPF()
     g.init(cfg)
     sc.build(g)
@@ -53,9 +54,10 @@ def retarget(g, sc, counters):
         g.set_agent_targets(done.astype(np.int32), tg)
 
 
-def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None):
-    """Continuous re-path frames; returns timing and counter statistics."""
-    g = make_world(sc, mode, T, Q, budget, profile)
+def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None, spw=0):
+    """Continuous re-path frames; returns timing and counter statistics.
+    `spw`: searches per search workgroup (0: 64 / T)."""
+    g = make_world(sc, mode, T, Q, budget, profile, spw=spw)
     counters = np.array(sc.counters, copy=True)
     frame_ms, found_d, exp_d, search_ms, host_ms = [], [], [], [], []
     prev = g.stats()

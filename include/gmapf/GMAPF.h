@@ -51,6 +51,7 @@ struct Config {
     float HashCell = 0;
     int Device = 0;
     long long CorridorSlab = 0;  // corridor storage in ints; 0: sized from agents and map
+    int SearchesPerWave = 0;     // searches per 64-lane search workgroup; 0: 64 / T
 };
 
 class CuNodesMap {
@@ -95,6 +96,7 @@ class GMAPF {
         cc.hash_cell = c.HashCell;
         cc.device = c.Device;
         cc.corridor_slab = c.CorridorSlab;
+        cc.searches_per_wave = c.SearchesPerWave;
         check(gmapf_init(h_, &cc));
     }
     void ManualStart() { check(gmapf_manual_start(h_)); }

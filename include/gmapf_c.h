@@ -59,6 +59,10 @@ typedef struct gmapf_config {
                                           16 * V)); at least V.  A full slab delays a found
                                           search's corridor to a later frame; compaction keeps
                                           only the corridors of moving agents */
+    int searches_per_wave;             /* searches a 64-lane search workgroup holds, 1 .. 64 / T;
+                                          <= 0: 64 / T.  Fewer than 64 / T leaves lanes idle
+                                          and puts more workgroups on each SIMD, which hides
+                                          memory latency when searches are too few to fill it */
 } gmapf_config;
 
 /* Reference CuNodesMap::Node (include/gmapf/misc/CuNodesMap.h:14-21);

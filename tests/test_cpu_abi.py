@@ -83,6 +83,17 @@ def test_call_order_errors():
     g.close()
 
 
+def test_searches_per_wave_bounded_by_lanes():
+    g = G.GMAPF()
+    g.init(G.Config(agents_max_count=1, path_finder_threads_per_agents=4, searches_per_wave=17))
+    tri, nbr = H.grid_map(1, 6, 6, cell=2.0, p=0.0)
+    g.fill_map_manually(tri, nbr)
+    with pytest.raises(G.GmapfError) as e:
+        g.manual_start()  # 4 lanes per search: at most 16 searches per 64-lane workgroup
+    assert e.value.code == -1
+    g.close()
+
+
 def test_bad_neighbour_index_rejected():
     g = G.GMAPF()
     g.init(G.Config(agents_max_count=1))

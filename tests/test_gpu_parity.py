@@ -12,17 +12,19 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("T,Q", [(2, 16), (4, 16), (8, 32), (16, 8), (32, 4), (64, 2), (4, 1)])
-def test_single_frame_search_parity(oracle, T, Q):
-    """One frame, every agent searches once; corridors and flags vs oracle.search."""
+@pytest.mark.parametrize("T,Q,spw", [(2, 16, 0), (4, 16, 0), (8, 32, 0), (16, 8, 0), (32, 4, 0), (64, 2, 0), (4, 1, 0),
+                                     (4, 16, 8), (2, 16, 7), (8, 16, 1)])
+def test_single_frame_search_parity(oracle, T, Q, spw):
+    """One frame, every agent searches once; corridors and flags vs
+    oracle.search.  spw > 0: fewer searches per workgroup than 64 / T."""
     tri, nbr = H.grid_map(11, 48, 40, cell=4.0, p=0.3)
     cen = S.centroids(tri)
     pairs = H.connected_pairs(5, nbr, 300)
     cfg = G.Config(agents_max_count=len(pairs), path_finder_parallel_agents=len(pairs),
                    path_finder_threads_per_agents=T, path_finder_queue_capacity=Q,
-                   agent_radius=0.25, agent_speed=1.0)
+                   agent_radius=0.25, agent_speed=1.0, searches_per_wave=spw)
     g = H.make_gpu(tri, nbr, cfg, [cen[a] for a, _ in pairs], [cen[b] for _, b in pairs])
-    print(f"search parity T={T} Q={Q} V={len(tri)}", flush=True)
+    print(f"search parity T={T} Q={Q} spw={spw} V={len(tri)}", flush=True)
     g.step(1.0 / 60)
     om = oracle.OracleMap(tri, nbr)
     _, st = g.agents()
@@ -38,7 +40,7 @@ def test_single_frame_search_parity(oracle, T, Q):
     assert nfound > 0
 
 
-def _world(seed, nx, ny, n_agents, T, Q, mode=G.MODE_PLAIN, budget=0, p=0.2, P=None):
+def _world(seed, nx, ny, n_agents, T, Q, mode=G.MODE_PLAIN, budget=0, p=0.2, P=None, spw=0):
     tri, nbr = H.grid_map(seed, nx, ny, cell=4.0, p=p)
     cen = S.centroids(tri)
     lab = S.components(nbr)
@@ -50,17 +52,19 @@ def _world(seed, nx, ny, n_agents, T, Q, mode=G.MODE_PLAIN, budget=0, p=0.2, P=N
     targs = [cen[nodes[r.randint(len(nodes))]] for _ in range(n_agents)]
     cfg = G.Config(agents_max_count=n_agents, path_finder_parallel_agents=P or n_agents,
                    path_finder_threads_per_agents=T, path_finder_queue_capacity=Q,
-                   agent_radius=0.5, agent_speed=2.0, step_mode=mode, round_budget=budget or 1)
+                   agent_radius=0.5, agent_speed=2.0, step_mode=mode, round_budget=budget or 1,
+                   searches_per_wave=spw)
     return tri, nbr, cfg, starts, targs
 
 
-@pytest.mark.parametrize("mode,budget,P", [(G.MODE_PLAIN, 0, None), (G.MODE_PLAIN, 0, 37),
-                                           (G.MODE_BATCHED, 3, None), (G.MODE_BATCHED, 1, 50)])
-def test_multi_frame_world_parity(oracle, mode, budget, P):
+@pytest.mark.parametrize("mode,budget,P,spw", [(G.MODE_PLAIN, 0, None, 0), (G.MODE_PLAIN, 0, 37, 0),
+                                               (G.MODE_BATCHED, 3, None, 0), (G.MODE_BATCHED, 1, 50, 0),
+                                               (G.MODE_BATCHED, 3, None, 3)])
+def test_multi_frame_world_parity(oracle, mode, budget, P, spw):
     """Agents search, move, avoid and progress for many frames; every frame
     must equal the oracle bit for bit (positions, states, nodes, corridors)."""
-    tri, nbr, cfg, starts, targs = _world(21, 40, 40, 160, 8, 16, mode, budget, P=P)
-    print(f"world parity mode={mode} budget={budget} P={P}", flush=True)
+    tri, nbr, cfg, starts, targs = _world(21, 40, 40, 160, 8, 16, mode, budget, P=P, spw=spw)
+    print(f"world parity mode={mode} budget={budget} P={P} spw={spw}", flush=True)
     g = H.make_gpu(tri, nbr, cfg, starts, targs)
     w = H.make_oracle(oracle, tri, nbr, cfg, starts, targs)
     dt = 1.0 / 60
