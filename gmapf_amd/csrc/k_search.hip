@@ -192,6 +192,58 @@ __device__ __forceinline__ unsigned claim_hash(int n, int bits) {
     return ((unsigned)n * 2654435761u) >> (32 - bits);
 }
 
+// First candidate of each of a lane's three neighbour slots among the
+// candidates of its segment that come earlier in (lane, slot) order.  A
+// candidate is (node << 1 | direction) when `ins` (the node exists and this
+// search has not marked it), else absent.  first[s] is the earliest earlier
+// candidate for the same node, or -1 when slot s's candidate is the node's
+// first.  Lanes l - 1 .. l - (T - 1) are read with DPP row shifts (a segment
+// of T <= 16 lanes lies inside one 16-lane row; the lanes of the previous
+// segment that a shift also reaches are masked by l >= n), nearest lane
+// first so that the earliest match is the one kept.
+template <int N>
+__device__ __forceinline__ int row_shr(int v) {
+    // row_shr:N; a lane with no source lane in its row gets -2 (absent)
+    return __builtin_amdgcn_update_dpp(-2, v, 0x110 + N, 0xf, 0xf, false);
+}
+
+template <int T, int N>
+__device__ __forceinline__ void seg_first_from(int l, const int* key, const int* pk, int* first) {
+    if constexpr (N < T) {
+        const bool v = l >= N;
+        int wp[3], ws[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            wp[s] = row_shr<N>(pk[s]);
+            ws[s] = v ? (wp[s] >> 1) : -1;  // absent: -1, never equal to a key
+        }
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            first[s] = key[s] == ws[2] ? wp[2] : first[s];
+            first[s] = key[s] == ws[1] ? wp[1] : first[s];
+            first[s] = key[s] == ws[0] ? wp[0] : first[s];
+        }
+        seg_first_from<T, N + 1>(l, key, pk, first);
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void seg_first_candidates(int l, const int* nn, const bool* ins, int cdir, int* first) {
+    int pk[3], key[3], own[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        pk[s] = ins[s] ? ((nn[s] << 1) | cdir) : -2;
+        own[s] = ins[s] ? nn[s] : -1;
+        key[s] = ins[s] ? nn[s] : -3;  // matches nothing when absent
+    }
+    // earlier slots of the lane itself
+    first[0] = -1;
+    first[1] = key[1] == own[0] ? pk[0] : -1;
+    first[2] = key[2] == own[1] ? pk[1] : -1;
+    first[2] = key[2] == own[0] ? pk[0] : first[2];
+    seg_first_from<T, 1>(l, key, pk, first);
+}
+
 // Corridor walk of one side of a found search (PathFinder.h:276-306), one
 // parent step per call so the walk of a search that ended overlaps the rounds
 // of the other searches of the workgroup.  The corridor occupies
@@ -522,59 +574,74 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         // request comes after the round's other loads, never before them)
         if (__ballot(walk.active) != 0ull && l < 2) walk_advance(walk, l, slab);
 
-        // ---- this round's candidates on unmarked nodes: place each node in
-        // the segment's table (linear probing, first compare-and-swap wins
-        // the slot), then keep the smallest (lane, slot) index per node
-        int* hk = &hkey[sub * HS];
-        int* hv = &hval[sub * HS];
-        unsigned hslot[3];
-        bool ins[3], pend[3];
-        int* kp[3];
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            ins[s] = nn[s] >= 0 && !prior[s];
-            hslot[s] = claim_hash(nn[s] >= 0 ? nn[s] : 0, HB);
-            kp[s] = ins[s] ? &hk[hslot[s]] : &hdummy[L];
-        }
-        {
-            int old[3];
-#pragma unroll
-            for (int s = 0; s < 3; ++s) old[s] = atomicCAS(kp[s], kHashEmpty, nn[s]);
-#pragma unroll
-            for (int s = 0; s < 3; ++s) pend[s] = ins[s] && old[s] != kHashEmpty && old[s] != nn[s];
-        }
-        // slot taken by another node: probe on (rare)
-        while (__ballot(pend[0] || pend[1] || pend[2]) != 0ull) {
-#pragma unroll
-            for (int s = 0; s < 3; ++s) {
-                if (pend[s]) {
-                    hslot[s] = (hslot[s] + 1u) & (unsigned)(HS - 1);
-                    const int old = atomicCAS(&hk[hslot[s]], kHashEmpty, nn[s]);
-                    pend[s] = old != kHashEmpty && old != nn[s];
-                }
-            }
-        }
-        int* vp[3];
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            vp[s] = ins[s] ? &hv[hslot[s]] : &hdummy[L];
-            atomicMin(vp[s], ((l * 3 + s) << 1) | cdir);
-        }
-        wave_sync();
-        int ev[3];
-#pragma unroll
-        for (int s = 0; s < 3; ++s) ev[s] = *vp[s];
-
         // ---- claims and meets.  A candidate claims an unmarked node when it
         // is the node's first candidate in (lane, slot) order; otherwise the
         // node's owner is the earlier marker (prior) or the first candidate,
         // and a different direction is a meet.
-        bool claim[3], meet[3];
+        bool ins[3], claim[3], meet[3];
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            claim[s] = ins[s] && (ev[s] >> 1) == l * 3 + s;
-            const int odir = ins[s] ? (ev[s] & 1) : pdir[s];
-            meet[s] = nn[s] >= 0 && !claim[s] && odir != cdir;
+        for (int s = 0; s < 3; ++s) ins[s] = nn[s] >= 0 && !prior[s];
+        [[maybe_unused]] unsigned hslot[3] = {0u, 0u, 0u};
+        if constexpr (T <= 4) {
+            // the segment's lanes compare their candidates directly: a
+            // segment of at most 16 lanes lies in one 16-lane DPP row, so a
+            // row shift by n brings lane l - n's candidates to lane l (at
+            // T = 8 the 63 compares per lane cost more than the table)
+            int first[3];
+            seg_first_candidates<T>(l, nn, ins, cdir, first);
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                claim[s] = ins[s] && first[s] < 0;
+                const int odir = ins[s] ? (first[s] >= 0 ? (first[s] & 1) : cdir) : pdir[s];
+                meet[s] = nn[s] >= 0 && !claim[s] && odir != cdir;
+            }
+        } else {
+            // place each node in the segment's table (linear probing, first
+            // compare-and-swap wins the slot), then keep the smallest (lane,
+            // slot) index per node
+            int* hk = &hkey[sub * HS];
+            int* hv = &hval[sub * HS];
+            bool pend[3];
+            int* kp[3];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                hslot[s] = claim_hash(nn[s] >= 0 ? nn[s] : 0, HB);
+                kp[s] = ins[s] ? &hk[hslot[s]] : &hdummy[L];
+            }
+            {
+                int old[3];
+#pragma unroll
+                for (int s = 0; s < 3; ++s) old[s] = atomicCAS(kp[s], kHashEmpty, nn[s]);
+#pragma unroll
+                for (int s = 0; s < 3; ++s) pend[s] = ins[s] && old[s] != kHashEmpty && old[s] != nn[s];
+            }
+            // slot taken by another node: probe on (rare)
+            while (__ballot(pend[0] || pend[1] || pend[2]) != 0ull) {
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    if (pend[s]) {
+                        hslot[s] = (hslot[s] + 1u) & (unsigned)(HS - 1);
+                        const int old = atomicCAS(&hk[hslot[s]], kHashEmpty, nn[s]);
+                        pend[s] = old != kHashEmpty && old != nn[s];
+                    }
+                }
+            }
+            int* vp[3];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                vp[s] = ins[s] ? &hv[hslot[s]] : &hdummy[L];
+                atomicMin(vp[s], ((l * 3 + s) << 1) | cdir);
+            }
+            wave_sync();
+            int ev[3];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) ev[s] = *vp[s];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                claim[s] = ins[s] && (ev[s] >> 1) == l * 3 + s;
+                const int odir = ins[s] ? (ev[s] & 1) : pdir[s];
+                meet[s] = nn[s] >= 0 && !claim[s] && odir != cdir;
+            }
         }
 
         // ---- claims: frontier order = (lane, slot) order
@@ -611,11 +678,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         }
         const unsigned long long pb = __ballot(popped) & smask;
         wave_sync();
-        // empty this round's table slots for the next round
+        if constexpr (T > 4) {
+            // empty this round's table slots for the next round
+            int* hk = &hkey[sub * HS];
+            int* hv = &hval[sub * HS];
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            *(ins[s] ? &hk[hslot[s]] : &hsink[L]) = kHashEmpty;
-            *(ins[s] ? &hv[hslot[s]] : &hsink[L]) = kHashNone;
+            for (int s = 0; s < 3; ++s) {
+                *(ins[s] ? &hk[hslot[s]] : &hsink[L]) = kHashEmpty;
+                *(ins[s] ? &hv[hslot[s]] : &hsink[L]) = kHashNone;
+            }
         }
 
         // ---- deal frontier item f to lane f mod T (at most three per lane),
