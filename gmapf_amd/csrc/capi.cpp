@@ -378,6 +378,8 @@ struct gmapf_s {
         dsl.status = dalloc<int>(P, "slot status");
         dsl.mS = dalloc<int>(P, "slot meet start side");
         dsl.mT = dalloc<int>(P, "slot meet target side");
+        dsl.corr = dalloc<int2>(P, "slot corridor in walk");
+        dsl.walk = dalloc<int4>(P, "slot parked walk");
         dsl.VP = ((long long)V + 63) / 64 * 64;
         dsl.mark = dalloc<unsigned char>((size_t)P * dsl.VP, "visited marks");
         dsl.vis = dalloc<uint2>((size_t)P * V, "visited entries");
@@ -412,8 +414,9 @@ struct gmapf_s {
         dfr.hscatter = dalloc<int>(Nn, "hash scatter");
         dfr.horder = dalloc<int>(Nn, "hash order");
         dfr.hblk = dalloc<int>(dfr.ncell / scan_block + 2, "hash scan partials");
-        dfr.slab_len = dalloc<int>((size_t)Nn + 1, "corridor lengths");
-        dfr.slab_blk = dalloc<int>(Nn / scan_block + 2, "corridor scan partials");
+        // corridors of agents, then of slots whose corridor walk is parked
+        dfr.slab_len = dalloc<int>((size_t)Nn + P + 1, "corridor lengths");
+        dfr.slab_blk = dalloc<int>((Nn + P) / scan_block + 2, "corridor scan partials");
         dfr.stepv = dalloc<float>(1, "step");
         HIPCHK(launch_fill(dfr.counters, 0, 16, s1));
         HIPCHK(launch_fill_u64(dfr.stats, 0, 8, s1));
@@ -465,7 +468,7 @@ struct gmapf_s {
     // ------------------------------------------------------------ frame
     void enqueue_frame(bool compact) {
         HIPCHK(launch_admit(dmap, dag, dsl, dfr, s1));
-        if (compact) HIPCHK(launch_compact(dag, dfr, s1));
+        if (compact) HIPCHK(launch_compact(dag, dsl, dfr, s1));
         HIPCHK(hipEventRecord(ev_fork, s1));
         HIPCHK(hipStreamWaitEvent(s2, ev_fork, 0));
         HIPCHK(launch_search(dmap, dag, dsl, dfr, sp, search_grid, persistent, s1));
@@ -486,7 +489,7 @@ struct gmapf_s {
     void enqueue_frame_profiled(bool compact) {
         HIPCHK(hipEventRecord(ev_p[0], s1));
         HIPCHK(launch_admit(dmap, dag, dsl, dfr, s1));
-        if (compact) HIPCHK(launch_compact(dag, dfr, s1));
+        if (compact) HIPCHK(launch_compact(dag, dsl, dfr, s1));
         if (cfg.is_debug) debug_sync("admit");
         HIPCHK(hipEventRecord(ev_p[1], s1));
         HIPCHK(launch_search(dmap, dag, dsl, dfr, sp, search_grid, persistent, s1));

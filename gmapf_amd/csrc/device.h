@@ -94,9 +94,13 @@ struct DevSlots {
     int* Tg;
     unsigned* stamp;
     int* status;    // 0 running, 1 found, 2 exhausted, 3 fresh (needs seeding),
-                    // 4 found, corridor deferred (needs slab space)
+                    // 4 found, corridor deferred (needs slab space),
+                    // 5 found, corridor walk parked (batched form)
     int* mS;        // meet of a found search: start-side and target-side nodes
     int* mT;
+    int2* corr;     // batched form, while the corridor is walked: its slab offset and length
+    int4* walk;     // parked walk: {node, index - offset} of the start side, then the
+                    // target side (node -1: side done)
     unsigned char* mark;    // P * VP  (m << 1) | dir
     uint2* vis;             // P * V   {G << 1 | dir, parent}
     long long VP;           // mark stride: V rounded up to 64

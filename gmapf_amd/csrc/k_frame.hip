@@ -399,38 +399,50 @@ __global__ void k_export(DevAgents ag, DevFrame fr) {
 }
 
 // ------------------------------------------------------------- corridor slab
-// live length per agent: a moving agent keeps its corridor, the others' are
-// released
-__global__ void k_slab_len(DevAgents ag, DevFrame fr) {
+// Live length per owner, owners 0 .. n-1 the agents and n .. n+P-1 the search
+// slots: a moving agent keeps its corridor, a slot keeps the corridor of its
+// parked walk (batched form; the walk stores its positions relative to the
+// corridor's offset), everything else is released.
+__global__ void k_slab_len(DevAgents ag, DevSlots sl, DevFrame fr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ag.n) fr.slab_len[i] = (ag.state[i] == kMove && ag.path_len[i] > 0) ? ag.path_len[i] : 0;
+    if (i < ag.n) {
+        fr.slab_len[i] = (ag.state[i] == kMove && ag.path_len[i] > 0) ? ag.path_len[i] : 0;
+    } else if (i < ag.n + sl.P) {
+        const int s = i - ag.n;
+        fr.slab_len[i] = (sl.status[s] == 5 && sl.agent[s] >= 0) ? sl.corr[s].y : 0;
+    }
 }
 
-// after the scan slab_len[i] is agent i's new offset and slab_len[n] the
-// total; one wave copies one agent's corridor
-__global__ void k_slab_copy(DevAgents ag, DevFrame fr) {
+// after the scan slab_len[o] is owner o's new offset and slab_len[n + P] the
+// total; one wave copies one owner's corridor
+__global__ void k_slab_copy(DevAgents ag, DevSlots sl, DevFrame fr) {
     const int* src = *ag.slab_now;
     int* dst = src == ag.slab[0] ? ag.slab[1] : ag.slab[0];
     const int lane = threadIdx.x % 64;
     const int w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
     const int nw = (gridDim.x * blockDim.x) / 64;
-    for (int a = w; a < ag.n; a += nw) {
+    for (int a = w; a < ag.n + sl.P; a += nw) {
         const int off = fr.slab_len[a];
         const int len = fr.slab_len[a + 1] - off;
-        const long long from = ag.path_off[a];
+        if (len == 0) continue;
+        const long long from = a < ag.n ? ag.path_off[a] : (long long)sl.corr[a - ag.n].x;
         for (int k = lane; k < len; k += 64) dst[off + k] = src[from + k];
         if (lane == 0) {
-            ag.path_off[a] = off;
-            ag.path_len[a] = len;
+            if (a < ag.n) {
+                ag.path_off[a] = off;
+                ag.path_len[a] = len;
+            } else {
+                sl.corr[a - ag.n].x = off;
+            }
         }
     }
 }
 
-__global__ void k_slab_finish(DevAgents ag, DevFrame fr) {
+__global__ void k_slab_finish(DevAgents ag, DevSlots sl, DevFrame fr) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int* cur = *ag.slab_now;
     *ag.slab_now = cur == ag.slab[0] ? ag.slab[1] : ag.slab[0];
-    *ag.slab_top = (unsigned long long)fr.slab_len[ag.n];
+    *ag.slab_top = (unsigned long long)fr.slab_len[ag.n + sl.P];
 }
 
 // ------------------------------------------------------------- crowd
@@ -613,13 +625,16 @@ hipError_t launch_export(const DevAgents& ag, const DevFrame& fr, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t launch_compact(const DevAgents& ag, const DevFrame& fr, hipStream_t st) {
+hipError_t launch_compact(const DevAgents& ag, const DevSlots& sl, const DevFrame& fr, hipStream_t st) {
+    const int owners = ag.n + sl.P;
     if (ag.n <= 0) return hipSuccess;
     hipError_t e;
-    hipLaunchKernelGGL(k_slab_len, dim3(nblk(ag.n, 256)), dim3(256), 0, st, ag, fr);
-    if ((e = exscan(fr.slab_len, ag.n, fr.slab_blk, fr.slab_len, &fr.slab_len[ag.n], st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_slab_copy, dim3(nblk(ag.n, 4) < 2048 ? nblk(ag.n, 4) : 2048), dim3(256), 0, st, ag, fr);
-    hipLaunchKernelGGL(k_slab_finish, dim3(1), dim3(64), 0, st, ag, fr);
+    hipLaunchKernelGGL(k_slab_len, dim3(nblk(owners, 256)), dim3(256), 0, st, ag, sl, fr);
+    if ((e = exscan(fr.slab_len, owners, fr.slab_blk, fr.slab_len, &fr.slab_len[owners], st)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_slab_copy, dim3(nblk(owners, 4) < 2048 ? nblk(owners, 4) : 2048), dim3(256), 0, st, ag, sl,
+                       fr);
+    hipLaunchKernelGGL(k_slab_finish, dim3(1), dim3(64), 0, st, ag, sl, fr);
     return hipGetLastError();
 }
 

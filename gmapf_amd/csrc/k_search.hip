@@ -49,6 +49,10 @@ namespace gmapf_amd {
 namespace {
 
 constexpr int kWave = 64;
+// s_waitcnt operand: vmcnt(0), expcnt and lgkmcnt not waited for (gfx9
+// layout: vmcnt[3:0] in bits 3:0 and [5:4] in bits 15:14, expcnt in 6:4,
+// lgkmcnt in 11:8)
+constexpr int kWaitVmem0 = 0x0f70;
 
 __device__ __forceinline__ unsigned long long make_key(float F, int node) {
     F = F + 0.0f;  // -0 -> +0 so equal floats give equal keys
@@ -292,14 +296,18 @@ __device__ __forceinline__ void walk_step(Walk& w, int side, global_int* slab, c
 
 // The corridor of a found search (meet mS, mT): lane 0 of the segment takes
 // G(mS) + G(mT) + 1 ints of the slab (PathFinder.h:280-290); when they fit,
-// the agent moves (state Move, path reset) and lanes 0 and 1 start their
-// walks, else the search keeps its slot with status 4 and tries again next
-// frame.  `share` is the segment's two LDS words (base or -1, G(mS)).  Called
-// by every lane of the segment; returns true when the corridor was placed.
+// lanes 0 and 1 start their walks, else the search keeps its slot with status
+// 4 and tries again next frame.  Plain form (`hold` false): the agent moves
+// now (state Move, path reset) and the walk runs beside later rounds.  Batched
+// form (`hold`): the segment keeps the slot and walks the corridor within its
+// round budget; the agent moves when the walk is done.  `share` is the
+// segment's three LDS words (base or -1, G(mS), length).  Called by every lane
+// of the segment; returns true when the corridor was placed, with its offset
+// and length in cbase, clen.
 __device__ __forceinline__ bool start_corridor(int l, int slot, int agent, int S, int Tg, int mS, int mT,
                                                const uint2* vis, global_int* slab, const DevAgents& ag,
-                                               const DevSlots& slots, size_t V, int* share, Walk& walk,
-                                               unsigned& st_found, unsigned& st_corr) {
+                                               const DevSlots& slots, size_t V, int* share, Walk& walk, bool hold,
+                                               int& cbase, int& clen, unsigned& st_found, unsigned& st_corr) {
     if (l < 2) {
         // a walk still pending on these lanes (their previous search in a
         // persistent segment) finishes first
@@ -312,7 +320,10 @@ __device__ __forceinline__ bool start_corridor(int l, int slot, int agent, int S
         const bool ok = o + (unsigned long long)len <= (unsigned long long)ag.slab_cap;
         share[0] = ok ? (int)o : -1;
         share[1] = gS;
-        if (ok) {
+        share[2] = len;
+        if (ok && hold) {
+            slots.corr[slot] = make_int2((int)o, len);
+        } else if (ok) {
             ag.path_off[agent] = (long long)o;
             ag.path_len[agent] = len;
             ag.state[agent] = kMove;
@@ -331,6 +342,8 @@ __device__ __forceinline__ bool start_corridor(int l, int slot, int agent, int S
     }
     wave_sync();
     const int base = share[0];
+    cbase = base;
+    clen = share[2];
     if (base >= 0 && l < 2) {
         const int gS = share[1];
         walk.active = true;
@@ -341,6 +354,16 @@ __device__ __forceinline__ bool start_corridor(int l, int slot, int agent, int S
         walk.ent = vis[walk.n];
     }
     return base >= 0;
+}
+
+// Batched form: lanes 0 and 1 store their side of a walk that stops at the
+// end of the launch (node -1: side done; position relative to the corridor's
+// offset, which compaction may move) and drop it.
+__device__ __forceinline__ void park_walk(int l, int slot, int cbase, const DevSlots& slots, Walk& walk) {
+    if (l < 2) {
+        reinterpret_cast<int2*>(&slots.walk[slot])[l] = make_int2(walk.active ? walk.n : -1, walk.idx - cbase);
+        walk.active = false;
+    }
 }
 
 // Resident waves per SIMD the launch needs to run one search per segment at
@@ -368,7 +391,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     // holds kHashEmpty, so a compare-and-swap on it never writes
     __shared__ int hdummy[kWave];
     __shared__ int hsink[kWave];  // per-lane target of the table resets of an empty slot
-    __shared__ int wshare[RPW * 2];  // per segment: corridor base (or -1) and G(mS)
+    __shared__ int wshare[RPW * 3];  // per segment: corridor base (or -1), G(mS), length
     for (int i = threadIdx.x; i < RPW * HS; i += kWave) {
         hkey[i] = kHashEmpty;
         hval[i] = kHashNone;
@@ -402,7 +425,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     int S = 0, Tg = 0, cnt = 0;
     unsigned mval = 0;  // this search's mark generation, 1..127
     float2 cS = make_float2(0.f, 0.f), cT = make_float2(0.f, 0.f);
-    int rounds_here = 0, exp_here = 0, disc_here = 0;
+    // rounds_here counts the segment's rounds in this launch (search and
+    // corridor walk, for the budget), srch_here its search rounds
+    int rounds_here = 0, srch_here = 0, exp_here = 0, disc_here = 0;
+    // batched form: the segment walks its search's corridor (cbase, clen)
+    bool walking = false;
+    int cbase = 0, clen = 0;
     int status = 0;
     uint2* vis = slots.vis;
     unsigned char* mark = slots.mark;
@@ -446,6 +474,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                 cS = map.cen[S];
                 cT = map.cen[Tg];
                 rounds_here = 0;
+                srch_here = 0;
+                walking = false;
                 exp_here = 0;
                 disc_here = 0;
                 vis = slots.vis + (size_t)slot * V;
@@ -457,9 +487,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                 has_req = true;
                 if (status == 4) {
                     // found earlier, corridor deferred for lack of slab space
-                    start_corridor(l, slot, agent, S, Tg, slots.mS[slot], slots.mT[slot], vis, slab, ag, slots, V,
-                                   &wshare[2 * sub], walk, st_found, st_corr);
-                    has_req = false;
+                    const bool hold = sp.budget > 0;
+                    const bool ok = start_corridor(l, slot, agent, S, Tg, slots.mS[slot], slots.mT[slot], vis, slab,
+                                                   ag, slots, V, &wshare[3 * sub], walk, hold, cbase, clen, st_found,
+                                                   st_corr);
+                    walking = hold && ok;
+                    has_req = walking;
+                    status = 0;
+                } else if (status == 5) {
+                    // batched form: a corridor walk parked at the end of the
+                    // last launch resumes
+                    const int4 pw = slots.walk[slot];
+                    const int2 cr = slots.corr[slot];
+                    cbase = cr.x;
+                    clen = cr.y;
+                    if (l < 2) {
+                        const int n = l == 0 ? pw.x : pw.z;
+                        walk.active = n >= 0;
+                        walk.slot = slot;
+                        walk.n = n;
+                        walk.term = l == 0 ? S : Tg;
+                        walk.idx = cbase + (l == 0 ? pw.y : pw.w);
+                        if (walk.active) walk.ent = vis[n];
+                    }
+                    walking = true;
+                    status = 0;
                 } else if (status == 3) {
                     if (mval == 1u) {
                         // generation wrapped (every 127 searches of the slot):
@@ -567,12 +619,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             prior[s] = nn[s] >= 0 && (mk[s] >> 1) == mval;
             pdir[s] = (int)(mk[s] & 1u);
         }
-        // one parent step of a pending corridor walk: the entry was
-        // requested at the end of the last round, right after the marks just
-        // used, so it has arrived with them (the vector memory counter is in
-        // order: a wait for a younger load waits for every older one, so the
-        // request comes after the round's other loads, never before them)
-        if (__ballot(walk.active) != 0ull && l < 2) walk_advance(walk, l, slab);
+        // corridor walks take two parent steps per round, each consuming an
+        // entry whose wait the round already pays: the vector memory counter
+        // is in order, so a wait for a load also covers every older one.
+        // Step 1 (here) consumes the entry requested at the end of the last
+        // round, after the marks just used; it requests the next entry,
+        // which is older than this round's record loads and so has arrived
+        // by the wait for them, where step 2 consumes it.  The marks are
+        // waited for explicitly before the request (vmcnt(0); the compiler
+        // would otherwise place their wait after the walk's code, where it
+        // would also wait for the load just requested).
+        __builtin_amdgcn_s_waitcnt(kWaitVmem0);
+        if (__ballot(walk.active) != 0ull && l < 2) {
+            walk_advance(walk, l, slab);
+            walk_request(walk, slots, V);
+        }
 
         // ---- claims and meets.  A candidate claims an unmarked node when it
         // is the node's first candidate in (lane, slot) order; otherwise the
@@ -667,7 +728,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         const int ms = meet[0] ? 0 : (meet[1] ? 1 : (meet[2] ? 2 : 3));
         const unsigned long long mball = __ballot(ms < 3);
         const unsigned long long mb = mball & smask;
-        const bool found = has_req && (mb != 0ull);
+        const bool found = has_req && !walking && (mb != 0ull);
         int mS = -1, mT = -1;
         if (mball != 0ull) {
             const int w = found ? __ffsll((long long)mb) - 1 : sub * T;
@@ -700,7 +761,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         // since every round ends with a pop).  When m is the top, the first
         // dealt item takes its place (one sift down instead of a push and a
         // pop).
-        const bool cont = has_req && !found;
+        const bool cont = has_req && !walking && !found;
         unsigned long long d0 = stage[segbase + l], d1 = stage[segbase + l + T], d2 = stage[segbase + l + 2 * T];
         d0 = cont && l < tot ? d0 : kKeyNone;
         d1 = cont && l + T < tot ? d1 : kKeyNone;
@@ -711,13 +772,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         m = d2 < m ? d2 : m;
         const bool has_next = m != kKeyNone;
         pf_c = has_next ? key_node(m) : -1;
-        pf_vn_ok = false;
-        if (has_next) {
-            pf_r0 = map.rec[3 * (size_t)pf_c];
-            pf_r1 = map.rec[3 * (size_t)pf_c + 1];
-            pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)pf_c + 2);
-            pf_vc = vis[pf_c];
-        }
+        // issued by every lane (node 0 of slot 0 when there is no next node:
+        // one shared cache line), so the loads and the wait for them below
+        // lie on every path and the compiler can count the loads in flight
+        const int pfa = has_next ? pf_c : 0;
+        const uint2* const pvis = has_next ? vis : slots.vis;
+        const unsigned char* const pmark = has_next ? mark : slots.mark;
+        pf_r0 = map.rec[3 * (size_t)pfa];
+        pf_r1 = map.rec[3 * (size_t)pfa + 1];
+        pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)pfa + 2);
+        pf_vc = pvis[pfa];
         {
             const bool from_top = has_next && m == top;
             // the root's replacement: the first dealt item, or the last item
@@ -731,14 +795,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             // marks of the next node's neighbours, as soon as its record is
             // in (it was requested before the sift): every write of this
             // round is visible (barrier above) and none can happen before the
-            // next round reads them
-            if (has_next) {
+            // next round reads them.  Every lane loads (node 0's neighbours
+            // when there is no next node).
+            {
                 const int a0 = pf_r0.x, a1 = a0 >= 0 ? pf_r0.y : -1, a2 = a1 >= 0 ? pf_r0.z : -1;
-                pf_vn[0] = mark[a0 >= 0 ? a0 : 0];
-                pf_vn[1] = mark[a1 >= 0 ? a1 : 0];
-                pf_vn[2] = mark[a2 >= 0 ? a2 : 0];
-                pf_vn_ok = true;
+                pf_vn[0] = pmark[a0 >= 0 ? a0 : 0];
+                pf_vn[1] = pmark[a1 >= 0 ? a1 : 0];
+                pf_vn[2] = pmark[a2 >= 0 ? a2 : 0];
+                pf_vn_ok = has_next;
             }
+            // corridor walk step 2 (the wait for the record covered its entry)
+            if (__ballot(walk.active) != 0ull && l < 2) walk_advance(walk, l, slab);
             // the items still to push, at most two: the dealt items other
             // than m (and d0 when it replaced the root)
             const bool skip0 = repl || m == d0;
@@ -751,14 +818,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         c_next = pf_c;
         if (l < 2) walk_request(walk, slots, V);
         const unsigned long long nb = __ballot(has_next) & smask;
+        const bool searching = has_req && !walking;
         rounds_here += has_req ? 1 : 0;
-        exp_here += has_req ? __popcll(pb) : 0;
-        disc_here += has_req ? tot : 0;
-        status = !has_req ? status : (found ? 1 : (nb == 0ull ? 2 : status));
-        const bool ended = has_req && status != 0;
+        srch_here += searching ? 1 : 0;
+        exp_here += searching ? __popcll(pb) : 0;
+        disc_here += searching ? tot : 0;
+        status = !searching ? status : (found ? 1 : (nb == 0ull ? 2 : status));
+        // batched form: a walking segment is done when both of its sides are
+        const bool wdone = walking && (__ballot(walk.active) & smask) == 0ull;
+        const bool ended = (searching && status != 0) || wdone;
         const bool park = has_req && !ended && sp.budget > 0 && rounds_here >= sp.budget;
         if (__ballot(ended || park) != 0ull && (ended || park)) {
-            if (park) {
+            if (park && !walking) {
                 // [node to expand next or kKeyNone, heap items]: at most Q keys
                 const size_t qb = ((size_t)slot * T + l);
                 slots.qcnt[qb] = cnt;
@@ -766,22 +837,56 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                 dst[0] = m;
                 for (int i = 0; i < cnt; ++i) dst[1 + i] = q[i * kWave + L];
             }
+            if (park && walking) park_walk(l, slot, cbase, slots, walk);
             if (l == 0) {
-                slots.status[slot] = status;
-                st_exp += (unsigned)exp_here;
-                st_disc += (unsigned)disc_here;
-                st_rounds += (unsigned)rounds_here;
-                if (ended && status == 2) {  // PathFinder.h:268-274
-                    ag.state[agent] = kIdle;
-                    ++st_fail;
-                    slots.agent[slot] = -1;
+                if (searching) {
+                    slots.status[slot] = status;
+                    st_exp += (unsigned)exp_here;
+                    st_disc += (unsigned)disc_here;
+                    st_rounds += (unsigned)srch_here;
+                    if (ended && status == 2) {  // PathFinder.h:268-274
+                        ag.state[agent] = kIdle;
+                        ++st_fail;
+                        slots.agent[slot] = -1;
+                        ag.inflight[agent] = -1;
+                    }
+                } else if (park) {
+                    slots.status[slot] = 5;
+                }
+                if (wdone) {  // batched form: corridor complete, the agent moves
+                    ag.path_off[agent] = (long long)cbase;
+                    ag.path_len[agent] = clen;
+                    ag.state[agent] = kMove;
+                    ag.step_idx[agent] = -1;
                     ag.inflight[agent] = -1;
+                    slots.agent[slot] = -1;
+                    slots.status[slot] = 1;
+                    ++st_found;
+                    st_corr += (unsigned)clen;
                 }
             }
-            if (ended && status == 1)
-                start_corridor(l, slot, agent, S, Tg, mS, mT, vis, slab, ag, slots, V, &wshare[2 * sub], walk,
-                               st_found, st_corr);
-            has_req = false;
+            bool walk_on = false;
+            if (searching && ended && status == 1) {
+                const bool hold = sp.budget > 0;
+                const bool ok = start_corridor(l, slot, agent, S, Tg, mS, mT, vis, slab, ag, slots, V,
+                                               &wshare[3 * sub], walk, hold, cbase, clen, st_found, st_corr);
+                if (hold && ok) {
+                    if (rounds_here >= sp.budget) {
+                        // budget spent in the meeting round: the walk starts
+                        // next launch
+                        park_walk(l, slot, cbase, slots, walk);
+                        if (l == 0) slots.status[slot] = 5;
+                    } else {
+                        walk_on = true;
+                    }
+                }
+            }
+            walking = walk_on;
+            has_req = walk_on;
+            if (walk_on) {
+                status = 0;
+                popped_next = false;
+            }
         }
     }
     // walks still pending when the last search of the workgroup ended: the

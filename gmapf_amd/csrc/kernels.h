@@ -29,7 +29,7 @@ hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& 
 hipError_t launch_export(const DevAgents& ag, const DevFrame& fr, hipStream_t st);
 // corridor slab compaction: corridors of moving agents into the other buffer
 // in agent order; the others are released
-hipError_t launch_compact(const DevAgents& ag, const DevFrame& fr, hipStream_t st);
+hipError_t launch_compact(const DevAgents& ag, const DevSlots& sl, const DevFrame& fr, hipStream_t st);
 // F4: move, hash + avoidance (Jacobi), progress; all no-ops when the moving
 // set of the frame is empty.
 hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& fr,

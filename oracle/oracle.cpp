@@ -241,6 +241,8 @@ struct Search {
     int mS = -1, mT = -1;
     long long rounds = 0, expansions = 0, discoveries = 0;
     std::vector<QItem> frontier;
+    // batched form: rounds of corridor walk still to run after the meet
+    long long walk_left = 0;
 };
 
 // PathFinder.h:168-189: lane 0 seeds the start, lane 1 the target.
@@ -308,6 +310,16 @@ bool search_round(const Map& m, Search& s) {
 
 // PathFinder.h:276-306: start side reversed (start excluded), then the target
 // side up to and including the target.
+// Batched form (no reference counterpart): the corridor is built by two parent
+// steps per round on each side, from the round after the meet; the start side
+// takes max(1, G(mS)) steps, the target side max(1, G(mT)) (its last step
+// also writes the target).  The agent moves once both sides are done.
+constexpr long long kWalkStepsPerRound = 2;
+long long walk_rounds(const Search& s) {
+    const long long a = std::max(1, s.vis.find(s.mS)->G), b = std::max(1, s.vis.find(s.mT)->G);
+    return (std::max(a, b) + kWalkStepsPerRound - 1) / kWalkStepsPerRound;
+}
+
 void build_corridor(const Search& s, std::vector<int>& path) {
     int lenS = s.vis.find(s.mS)->G;
     int lenT = s.vis.find(s.mT)->G + 1;
@@ -569,23 +581,41 @@ void world_step(World& w, float dt) {
     std::vector<int> M;
     for (int i = 0; i < N; i++)
         if (w.agents[i].state == MOVE) M.push_back(i);
-    // F3: PathFinder.h:446-456 (+ round budget for the batched form)
-    long long budget = w.cfg.round_budget > 0 ? w.cfg.round_budget : (1LL << 62);
+    // F3: PathFinder.h:446-456 (+ round budget for the batched form, where a
+    // found search also walks its corridor within the budget)
+    const bool batched = w.cfg.round_budget > 0;
+    long long budget = batched ? w.cfg.round_budget : (1LL << 62);
     for (auto*& s : w.searches) {
         bool ended = false;
-        long long e0 = s->expansions, d0 = s->discoveries, r0 = s->rounds;
-        for (long long r = 0; r < budget; r++) {
-            if (search_round(m, *s)) {
-                ended = true;
-                break;
+        long long used = 0;
+        if (s->walk_left > 0) {  // corridor walk carried over from an earlier frame
+            used = std::min(s->walk_left, budget);
+            s->walk_left -= used;
+            if (s->walk_left > 0) continue;
+            ended = true;
+        } else {
+            long long e0 = s->expansions, d0 = s->discoveries, r0 = s->rounds;
+            for (long long r = 0; r < budget; r++) {
+                used++;
+                if (search_round(m, *s)) {
+                    ended = true;
+                    break;
+                }
+            }
+            // work is counted in the frame it is done (also for searches
+            // that are later dropped or still running)
+            w.st[2] += s->expansions - e0;
+            w.st[3] += s->discoveries - d0;
+            w.st[5] += s->rounds - r0;
+            if (!ended) continue;
+            if (batched && s->found) {
+                const long long need = walk_rounds(*s);
+                if (need > budget - used) {
+                    s->walk_left = need - (budget - used);
+                    continue;
+                }
             }
         }
-        // work is counted in the frame it is done (also for searches that
-        // are later dropped or still running)
-        w.st[2] += s->expansions - e0;
-        w.st[3] += s->discoveries - d0;
-        w.st[5] += s->rounds - r0;
-        if (!ended) continue;
         Agent& a = w.agents[s->agent];
         if (s->found) {  // PathFinder.h:280-290
             build_corridor(*s, a.path);
