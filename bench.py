@@ -70,34 +70,18 @@ def measured_traffic(T, Q, budget, n):
     return None
 
 
-def roofline_search(sc, T, Q, budget, frames=6):
-    """Search-kernel HBM roofline: algorithmic bytes 48*E + 16*D + 8*L per
-    launch (E expansions, D discoveries, L corridor nodes; device counters)
-    over the kernel's HIP-event duration on its own stream (profiled batched
-    frames, same kernels as the captured graph)."""
-    g = make_world(sc, G.MODE_BATCHED, T, Q, budget, profile=True)
-    counters = np.array(sc.counters, copy=True)
-    prev = g.stats()
-    tot_bytes, tot_ms = 0.0, 0.0
-    for f in range(frames):
-        g.async_step(sc.dt)
-        g.wait_step_end()
-        st = g.stats()
-        ms = g.last_frame_ms()
-        if f >= 2:
-            E = st["expansions"] - prev["expansions"]
-            D = st["discoveries"] - prev["discoveries"]
-            L = st["corridor_nodes"] - prev["corridor_nodes"]
-            tot_bytes += 48.0 * E + 16.0 * D + 8.0 * L
-            tot_ms += ms[2]
-        prev = st
-        retarget(g, sc, counters)
-    g.close()
-    n = max(1, frames - 2)
-    gbs = tot_bytes / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else 0.0
+def roofline_search(bat, T, Q, budget, n):
+    """Search-kernel HBM roofline over the timed batched frames: algorithmic
+    bytes 48*E + 16*D + 8*L per launch (E expansions, D discoveries, L
+    corridor nodes; device counters, frame by frame) over the kernel's
+    HIP-event time on its own stream (events recorded around the launch in
+    the captured frame graph), averaged over the same frames."""
+    ms = bat["search_ms"]
+    b = bat["search_bytes"]
+    gbs = b / (ms / 1e3) / 1e9 if ms > 0 else 0.0
     return dict(bound="hbm", achieved=round(float(gbs), 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(float(gbs) / HBM_PEAK_GBS, 5), traffic=measured_traffic(T, Q, budget, sc.n),
-                bytes_per_launch=tot_bytes / n, ms_per_launch=tot_ms / n)
+                frac=round(float(gbs) / HBM_PEAK_GBS, 5), traffic=measured_traffic(T, Q, budget, n),
+                bytes_per_launch=b, ms_per_launch=ms)
 
 
 def cpu_baseline(sc, loc, T, Q, seconds=12.0):
@@ -351,7 +335,7 @@ def main():
                        plain_solves_per_s=round(fp["solves_per_s"], 1) if fp else None,
                        batched_over_plain=round(fb["solves_per_s"] / fp["solves_per_s"], 3) if fp else None)
         log("fastest", fastest)
-    roof = roofline_search(sc, T, Q, B)
+    roof = roofline_search(bat, T, Q, B, sc.n)
     log("roofline", roof)
     cpu = None if args.skip_cpu else cpu_baseline(sc, loc, T, Q)
     out = {

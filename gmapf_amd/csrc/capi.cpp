@@ -69,6 +69,10 @@ struct gmapf_s {
     hipStream_t s1 = nullptr, s2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_p[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // around the search launch of every frame (recorded into the captured
+    // frame graph too): the search kernel's time, frame by frame
+    hipEvent_t ev_s[2] = {nullptr, nullptr};
+    bool capturing = false;  // enqueue_frame runs under stream capture
     DevMap dmap{};
     DevAgents dag{};
     DevSlots dsl{};
@@ -150,6 +154,8 @@ struct gmapf_s {
         h_upd_xy = nullptr;
         for (auto& e : ev_p)
             if (e) hipEventDestroy(e);
+        for (auto& e : ev_s)
+            if (e) hipEventDestroy(e);
         if (ev_fork) hipEventDestroy(ev_fork);
         if (ev_join) hipEventDestroy(ev_join);
         if (ev_done) hipEventDestroy(ev_done);
@@ -222,6 +228,7 @@ struct gmapf_s {
         HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming));
         for (auto& e : ev_p) HIPCHK(hipEventCreate(&e));
+        for (auto& e : ev_s) HIPCHK(hipEventCreate(&e));
 
         // ---- navmesh
         std::vector<int4> nbr4(V);
@@ -471,7 +478,13 @@ struct gmapf_s {
         if (compact) HIPCHK(launch_compact(dag, dsl, dfr, s1));
         HIPCHK(hipEventRecord(ev_fork, s1));
         HIPCHK(hipStreamWaitEvent(s2, ev_fork, 0));
+        // inside a stream capture the records become external event nodes
+        // of the graph (a plain record there only orders the capture and
+        // leaves the events without timestamps)
+        const unsigned evf = capturing ? hipEventRecordExternal : hipEventRecordDefault;
+        HIPCHK(hipEventRecordWithFlags(ev_s[0], s1, evf));
         HIPCHK(launch_search(dmap, dag, dsl, dfr, sp, search_grid, persistent, s1));
+        HIPCHK(hipEventRecordWithFlags(ev_s[1], s1, evf));
         HIPCHK(launch_crowd(dmap, dag, dfr, cp, s2));
         HIPCHK(hipEventRecord(ev_join, s2));
         HIPCHK(hipStreamWaitEvent(s1, ev_join, 0));
@@ -534,8 +547,11 @@ struct gmapf_s {
             if (!ge) {
                 HIPCHK(hipStreamBeginCapture(s1, hipStreamCaptureModeThreadLocal));
                 try {
+                    capturing = true;
                     enqueue_frame(compact);
+                    capturing = false;
                 } catch (...) {
+                    capturing = false;
                     hipGraph_t g = nullptr;
                     hipStreamEndCapture(s1, &g);
                     if (g) hipGraphDestroy(g);
@@ -591,6 +607,10 @@ struct gmapf_s {
                 HIPCHK(hipEventElapsedTime(&ms, ev_p[k], ev_p[k + 1]));
                 last_ms[k + 1] = ms;
             }
+        } else {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev_s[0], ev_s[1]));
+            last_ms[2] = ms;
         }
         for (int k = 0; k < 8; k++) {
             sum_ms[k] += last_ms[k];

@@ -59,7 +59,7 @@ def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None, s
     `spw`: searches per search workgroup (0: 64 / T)."""
     g = make_world(sc, mode, T, Q, budget, profile, spw=spw)
     counters = np.array(sc.counters, copy=True)
-    frame_ms, found_d, exp_d, search_ms, host_ms = [], [], [], [], []
+    frame_ms, found_d, exp_d, disc_d, corr_d, search_ms, host_ms = [], [], [], [], [], [], []
     prev = g.stats()
     for f in range(warmup + steps):
         g.async_step(sc.dt)
@@ -72,6 +72,8 @@ def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None, s
             host_ms.append(ms[5:8])
             found_d.append(st["found"] - prev["found"])
             exp_d.append(st["expansions"] - prev["expansions"])
+            disc_d.append(st["discoveries"] - prev["discoveries"])
+            corr_d.append(st["corridor_nodes"] - prev["corridor_nodes"])
         prev = st
         retarget(g, sc, counters)
         if log and (f == warmup - 1 or (f >= warmup and (f - warmup) % 30 == 29)):
@@ -86,7 +88,10 @@ def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None, s
         p50_ms=float(np.percentile(fm, 50)), p99_ms=float(np.percentile(fm, 99)),
         mean_ms=float(fm.mean()), found=int(np.sum(found_d)), expansions=int(np.sum(exp_d)),
         expansions_per_s=float(np.sum(exp_d) / tot_s) if tot_s > 0 else 0.0,
-        search_ms=float(np.mean(search_ms)) if profile else None,
+        # the search kernel's HIP-event time per frame, and its algorithmic
+        # bytes (48 per expansion, 16 per discovery, 8 per corridor node)
+        search_ms=float(np.mean(search_ms)),
+        search_bytes=float(np.mean(48.0 * np.array(exp_d) + 16.0 * np.array(disc_d) + 8.0 * np.array(corr_d))),
         host_enqueue_wait_readback_ms=[round(float(x), 4) for x in np.mean(np.array(host_ms), axis=0)],
         not_found=final["not_found"], stats=final, launch=info)
 

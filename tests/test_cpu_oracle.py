@@ -163,6 +163,43 @@ def test_path_graph_closed_form(oracle, n):
         assert st["length"] == len(c)
 
 
+@pytest.mark.parametrize("B", [0, 1, 2, 3, 5, 8, 1000])
+def test_batched_corridor_walk_closed_form(oracle, B):
+    """Batched form (budget B > 0): after the meet the corridor is walked at
+    two parent steps per round on each side, inside the budget, carried over
+    frames, and the agent moves when both sides are done.  On the path 0..n
+    with T=2 the meet takes R = ceil(n/2) rounds at mS = floor(n/2),
+    mT = mS + 1, so G(mS) = floor(n/2) and G(mT) = ceil(n/2) - 1; the walk
+    takes W = ceil(max(1, G(mS)) / 2) rounds and the agent first shows Move
+    after frame ceil((R + W) / B) - 1.  The plain form (B = 0) moves it after
+    frame 0 whatever n."""
+    for n in range(1, 41):
+        tri, nbr = path_graph(n)
+        om = oracle.OracleMap(tri, nbr)
+        f, c, st = om.search(0, n, 2, 4, 1.0)
+        R = math.ceil(n / 2)
+        assert f and st["rounds"] == R and st["mS"] == n // 2 and st["mT"] == n // 2 + 1
+        W = math.ceil(max(1, n // 2) / 2)
+        expect = 0 if B == 0 else math.ceil((R + W) / B) - 1
+        w = oracle.OracleWorld(om, agents_max=1, radius=0.1, speed=1.0, parallel=1, threads=2, queue_cap=4,
+                               round_budget=B)
+        cen = om.centroids()
+        assert w.add_agent(*cen[0]) == 0
+        assert w.set_target(0, *cen[n]) == 0
+        first = None
+        for frame in range(expect + 2):
+            w.step(1.0 / 60)
+            _, state, _ = w.agents()
+            if state[0] == 2:  # MOVE
+                first = frame
+                break
+            assert state[0] == 1, (n, B, frame)  # SEARCH until the corridor is complete
+        assert first == expect, (n, B, first, expect)
+        assert list(w.corridor(0)) == list(range(1, n + 1))
+        s = w.stats()
+        assert s["found"] == 1 and s["rounds"] == R and s["corridor_nodes"] == n
+
+
 def test_tree_corridor_is_the_unique_path(oracle):
     """In a tree every found corridor is the unique tree path; with Q >= V
     nothing is dropped, so every search must succeed."""
