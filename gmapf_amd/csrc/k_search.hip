@@ -36,9 +36,15 @@
 // its neighbours, are requested while the current round's frontier is
 // pushed.
 //
-// Forms: plain (one search per segment, run to completion) and persistent
-// (segments take searches from an atomic ticket over the active list, run up
-// to `budget` rounds, park their queues in HBM and take the next ticket).
+// Forms: plain (one search per segment, run to completion; the agent of a
+// found search moves at once and its corridor is walked beside the other
+// searches, finished lane-parallel at the end) and persistent (segments take
+// searches from an atomic ticket over the active list, or one each when the
+// grid holds a segment per search, run up to `budget` rounds, park their
+// queues in HBM and take the next ticket).  In the persistent form a found
+// search keeps its segment and walks its corridor inside the same budget,
+// two parent steps per round, parking the walk in its slot when the budget
+// runs out; the agent moves when the corridor is complete.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
