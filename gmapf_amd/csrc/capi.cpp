@@ -441,6 +441,8 @@ struct gmapf_s {
         HIPCHK(hipHostMalloc((void**)&h_upd_xy, (size_t)upd_cap * sizeof(float2)));
         h_upd[0] = 0;
         *h_step = 0.f;
+        // the crowd's last kernel writes only the positions it sets
+        if (N > 0) std::memcpy(h_pos_stage, pos.data(), (size_t)N * sizeof(float2));
         dfr.h_step = h_step;
         dfr.h_upd_n = h_upd;
         dfr.h_upd_ids = h_upd + 1;

@@ -367,18 +367,20 @@ __global__ void k_admit_slots(DevAgents ag, DevSlots sl, DevFrame fr) {
 // An agent whose search is in flight drops it: its slot is free for this
 // frame's admission, and the agent requests again with the new target.
 __global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
-    __shared__ int n_sh;
-    if (threadIdx.x == 0) {
-        n_sh = *fr.h_upd_n;
-        if (blockIdx.x == 0) *fr.stepv = *fr.h_step;
-    }
-    __syncthreads();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_sh) return;
-    const int i = fr.h_upd_ids[k];
+    if (k == 0) *fr.stepv = *fr.h_step;
+    // the count and this thread's entry are read together, one trip to host
+    // memory (the buffers hold one entry per agent; entries past the count
+    // are stale and unused)
+    const bool in = k < ag.n;
+    const int n = *fr.h_upd_n;
+    const int i = in ? fr.h_upd_ids[k] : 0;
+    const int tg = in ? fr.h_upd_targ[k] : 0;
+    const float2 xy = in ? fr.h_upd_xy[k] : make_float2(0.f, 0.f);
+    if (!in || k >= n) return;
     ag.state[i] = kSearch;
-    ag.targ[i] = fr.h_upd_targ[k];
-    ag.targ_pos[i] = fr.h_upd_xy[k];
+    ag.targ[i] = tg;
+    ag.targ_pos[i] = xy;
     ag.gen[i] = ag.gen[i] + 1;
     const int s = ag.inflight[i];
     if (s >= 0) {
@@ -387,13 +389,16 @@ __global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
     }
 }
 
-// positions, states and counters of the finished frame into pinned host memory
+// States and counters of the finished frame into pinned host memory.  The
+// positions are written there by the crowd's last kernel (the only one that
+// moves agents; it runs beside the search), so this pass after the search
+// writes 4 bytes per agent.  No kernel of the search writes host memory: a
+// store there inside the round loop would be waited for by the next round's
+// wait for all memory operations (the counter is in order), and the stores
+// would also hold up the end of the launch.
 __global__ void k_export(DevAgents ag, DevFrame fr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ag.n) {
-        fr.h_pos_out[i] = ag.pos[i];
-        fr.h_state_out[i] = ag.state[i];
-    }
+    if (i < ag.n) fr.h_state_out[i] = ag.state[i];
     if (i < 8) fr.h_stats_out[i] = fr.stats[i];
     if (i < 2) fr.h_slab_out[i] = ag.slab_top[i];  // [0] top, [1] deferred (adjacent words)
 }
@@ -558,6 +563,7 @@ __global__ void k_progress(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp
     if (i >= ag.n || fr.counters[5] == 0) return;
     const float2 p = ag.pos_tmp[i];
     ag.pos[i] = p;
+    fr.h_pos_out[i] = p;
     if (!ag.in_m[i]) return;
     const int* path = *ag.slab_now + ag.path_off[i];
     int idx = ag.step_idx[i];
