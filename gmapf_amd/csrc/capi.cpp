@@ -80,6 +80,7 @@ struct gmapf_s {
     SearchParams sp{};
     CrowdParams cp{};
     int search_grid = 0;
+    bool ident = false;  // slot i = agent i (P >= N)
     int persistent = 0;
     std::vector<void*> dallocs;
     // pinned staging
@@ -455,19 +456,24 @@ struct gmapf_s {
 
         // ---- search launch shape
         const int rpw = cfg.searches_per_wave > 0 ? cfg.searches_per_wave : 64 / T;
-        sp = SearchParams{T, Q, cfg.path_finder_heuristic_k, 0, rpw};
+        // P >= N: every request of a frame is admitted, so agent i can keep
+        // slot i and admission needs no ranks (the slots are interchangeable:
+        // a search's result does not depend on its slot)
+        ident = P >= N && N > 0;
+        sp = SearchParams{T, Q, cfg.path_finder_heuristic_k, 0, rpw, ident ? N : 0};
+        const int Pe = ident ? N : P;  // slots that can be in flight
         HIPCHK(search_prepare(T, Q));
         if (cfg.step_mode == GMAPF_MODE_BATCHED) {
             int per_cu = 0, cus = 0;
             HIPCHK(search_occupancy(T, Q, &per_cu));
             HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
             if (per_cu < 1) throw ApiError(GMAPF_E_ARG, "queue capacity does not fit in LDS");
-            int waves_needed = (P + rpw - 1) / rpw;
+            int waves_needed = (Pe + rpw - 1) / rpw;
             search_grid = std::max(1, std::min(per_cu * cus, waves_needed));
             persistent = 1;
             sp.budget = cfg.round_budget;
         } else {
-            search_grid = (P + rpw - 1) / rpw;
+            search_grid = (Pe + rpw - 1) / rpw;
             persistent = 0;
             sp.budget = 0;
         }
@@ -476,7 +482,7 @@ struct gmapf_s {
 
     // ------------------------------------------------------------ frame
     void enqueue_frame(bool compact) {
-        HIPCHK(launch_admit(dmap, dag, dsl, dfr, s1));
+        HIPCHK(launch_admit(dmap, dag, dsl, dfr, ident, s1));
         if (compact) HIPCHK(launch_compact(dag, dsl, dfr, s1));
         HIPCHK(hipEventRecord(ev_fork, s1));
         HIPCHK(hipStreamWaitEvent(s2, ev_fork, 0));
@@ -503,7 +509,7 @@ struct gmapf_s {
 
     void enqueue_frame_profiled(bool compact) {
         HIPCHK(hipEventRecord(ev_p[0], s1));
-        HIPCHK(launch_admit(dmap, dag, dsl, dfr, s1));
+        HIPCHK(launch_admit(dmap, dag, dsl, dfr, ident, s1));
         if (compact) HIPCHK(launch_compact(dag, dsl, dfr, s1));
         if (cfg.is_debug) debug_sync("admit");
         HIPCHK(hipEventRecord(ev_p[1], s1));

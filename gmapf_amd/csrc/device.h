@@ -159,6 +159,8 @@ struct SearchParams {
     float K;
     int budget;  // max rounds per search per launch; <= 0: unbounded
     int seg;     // searches per workgroup (segments 0 .. seg-1 take work), 1 .. 64 / T
+    int ident;   // > 0: every agent has a slot of its own (slot i = agent i, P >= N);
+                 // segment i takes slot i when it is in flight, and ident is the slot count
 };
 
 struct CrowdParams {

@@ -342,6 +342,47 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_scatter(DevAgents ag, D
     }
 }
 
+// Admission when every agent has a slot of its own (P >= N): every request
+// is admitted (PathFinder.h:85-116 with no deferral), into slot i for agent i
+// (PathFinder.h:104-115); the moving set is taken as in k_admit_count.  One
+// pass, no ranks; the counts go to counters[5], counters[6] (cleared by
+// k_apply_updates) and stats[6].
+__global__ __launch_bounds__(kAdmitThreads) void k_admit_identity(DevAgents ag, DevSlots sl, DevFrame fr) {
+    __shared__ int wsum_m[kAdmitWaves], wsum_a[kAdmitWaves];
+    const int i = blockIdx.x * kAdmitThreads + threadIdx.x;
+    bool mv = false, adm = false;
+    if (i < ag.n) {
+        const int st = ag.state[i];
+        mv = st == kMove;
+        ag.in_m[i] = mv ? 1 : 0;
+        if (st == kSearch && ag.inflight[i] < 0) {
+            if (ag.curr[i] == ag.targ[i]) {
+                ag.state[i] = kIdle;
+            } else {
+                sl.agent[i] = i;
+                sl.gen[i] = ag.gen[i];
+                sl.S[i] = ag.curr[i];
+                sl.Tg[i] = ag.targ[i];
+                sl.stamp[i] = sl.stamp[i] + 1u;
+                sl.status[i] = 3;
+                ag.inflight[i] = i;
+                ag.path_len[i] = 0;  // old corridor released (PathFinder.h:104-108)
+                adm = true;
+            }
+        }
+    }
+    int nm, na;
+    admit_rank(mv, wsum_m, &nm);
+    admit_rank(adm, wsum_a, &na);
+    if (threadIdx.x == 0) {
+        if (nm) atomicAdd(&fr.counters[5], nm);
+        if (na) {
+            atomicAdd(&fr.counters[6], na);
+            atomicAdd(&fr.stats[6], (unsigned long long)na);
+        }
+    }
+}
+
 // k-th admission: agent adm_agent[k] into slot free_list[k] (PathFinder.h:104-115)
 __global__ void k_admit_slots(DevAgents ag, DevSlots sl, DevFrame fr) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -368,7 +409,11 @@ __global__ void k_admit_slots(DevAgents ag, DevSlots sl, DevFrame fr) {
 // frame's admission, and the agent requests again with the new target.
 __global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) *fr.stepv = *fr.h_step;
+    if (k == 0) {
+        *fr.stepv = *fr.h_step;
+        fr.counters[5] = 0;  // counted by the admission pass that follows
+        fr.counters[6] = 0;
+    }
     // the count and this thread's entry are read together, one trip to host
     // memory (the buffers hold one entry per agent; entries past the count
     // are stale and unused)
@@ -616,10 +661,14 @@ hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, long lon
 }
 
 hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& sl,
-                        const DevFrame& fr, hipStream_t st) {
+                        const DevFrame& fr, bool ident, hipStream_t st) {
     (void)map;
     const int nbA = nblk(ag.n, kAdmitThreads), nbS = nblk(sl.P, kAdmitThreads);
     hipLaunchKernelGGL(k_apply_updates, dim3(nblk(ag.n > 0 ? ag.n : 1, 256)), dim3(256), 0, st, ag, sl, fr);
+    if (ident) {
+        hipLaunchKernelGGL(k_admit_identity, dim3(nbA > 0 ? nbA : 1), dim3(kAdmitThreads), 0, st, ag, sl, fr);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_admit_count, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
     hipLaunchKernelGGL(k_admit_scatter, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
     hipLaunchKernelGGL(k_admit_slots, dim3(nblk(sl.P, 256)), dim3(256), 0, st, ag, sl, fr);

@@ -414,8 +414,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     const int depth = heap_depth(Q);
     for (int r = 0; r < kPadRows; ++r) q[(Q + r) * kWave + threadIdx.x] = kKeyNone;
     const size_t V = (size_t)map.V;
-    const int n_active = fr.counters[1];
-    const bool one_each = (long long)n_active <= (long long)gridDim.x * sp.seg;
+    const int n_active = sp.ident > 0 ? sp.ident : fr.counters[1];
+    const bool one_each = sp.ident > 0 || (long long)n_active <= (long long)gridDim.x * sp.seg;
     const long long stride = fr.counters[9];
     // corridor slab in use (compaction runs between launches)
     global_int* const slab = (global_int*)*ag.slab_now;
@@ -468,10 +468,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                 item = blockIdx.x * sp.seg + sub;
                 took_plain = true;
             }
+            if (sp.ident > 0 && item >= 0 && item < n_active && slots.agent[item] < 0) item = -1;  // slot not in flight
             if (item < 0 || item >= n_active) {
                 no_more = true;
             } else {
-                slot = fr.active[(int)(((long long)item * stride) % n_active)];
+                slot = sp.ident > 0 ? item : fr.active[(int)(((long long)item * stride) % n_active)];
                 agent = slots.agent[slot];
                 S = slots.S[slot];
                 Tg = slots.Tg[slot];

@@ -23,8 +23,8 @@ hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, long lon
 // F0 + F1 + F2: target updates from pinned host memory, then cancel stale
 // searches, admit requests in ascending agent index into free slots in
 // ascending slot index, moving set, active list (one workgroup).
-hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& slots,
-                        const DevFrame& fr, hipStream_t st);
+hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& slots, const DevFrame& fr,
+                        bool ident, hipStream_t st);
 // positions, states and counters into pinned host memory
 hipError_t launch_export(const DevAgents& ag, const DevFrame& fr, hipStream_t st);
 // corridor slab compaction: corridors of moving agents into the other buffer
