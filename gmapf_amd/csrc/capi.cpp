@@ -523,6 +523,27 @@ struct gmapf_s {
         HIPCHK(launch_export(dag, dfr, s1));
     }
 
+    // Captures one frame (with or without the slab compaction in front) into
+    // a graph and instantiates it.
+    void build_graph(bool compact) {
+        hipGraph_t& gr = compact ? graph_c : graph;
+        hipGraphExec_t& ge = compact ? gexec_c : gexec;
+        HIPCHK(hipStreamBeginCapture(s1, hipStreamCaptureModeThreadLocal));
+        try {
+            capturing = true;
+            enqueue_frame(compact);
+            capturing = false;
+        } catch (...) {
+            capturing = false;
+            hipGraph_t g = nullptr;
+            hipStreamEndCapture(s1, &g);
+            if (g) hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(s1, &gr));
+        HIPCHK(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+    }
+
     void async_step(float dt) {
         require_init();
         if (step_going) return;  // GMAPF.cu:260-261
@@ -550,25 +571,11 @@ struct gmapf_s {
         const bool compact = compact_next;
         if (compact) compactions++;
         if (cfg.step_mode == GMAPF_MODE_BATCHED && !cfg.is_profiler && !cfg.is_debug) {
-            hipGraph_t& gr = compact ? graph_c : graph;
-            hipGraphExec_t& ge = compact ? gexec_c : gexec;
-            if (!ge) {
-                HIPCHK(hipStreamBeginCapture(s1, hipStreamCaptureModeThreadLocal));
-                try {
-                    capturing = true;
-                    enqueue_frame(compact);
-                    capturing = false;
-                } catch (...) {
-                    capturing = false;
-                    hipGraph_t g = nullptr;
-                    hipStreamEndCapture(s1, &g);
-                    if (g) hipGraphDestroy(g);
-                    throw;
-                }
-                HIPCHK(hipStreamEndCapture(s1, &gr));
-                HIPCHK(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
-            }
-            HIPCHK(hipGraphLaunch(ge, s1));
+            // both frame variants are captured on the first step, so a later
+            // frame never pays for an instantiation
+            if (!gexec) build_graph(false);
+            if (!gexec_c) build_graph(true);
+            HIPCHK(hipGraphLaunch(compact ? gexec_c : gexec, s1));
         } else if (cfg.is_profiler || cfg.is_debug) {
             enqueue_frame_profiled(compact);
         } else {
