@@ -799,6 +799,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             const unsigned long long last = q[lrow * kWave + L];
             q[lrow * kWave + L] = kKeyNone;  // the vacated entry
             heap_sift_down(q, L, Q, depth, repl || (shrink && cnt > 0), repl ? d0 : last);
+            // corridor walk step 2: its entry (requested in step 1, before
+            // the record loads) has arrived by now; the next entry is
+            // requested before the mark loads, so the wait for the marks at
+            // the top of the next round covers it without waiting longer
+            if (__ballot(walk.active) != 0ull && l < 2) {
+                walk_advance(walk, l, slab);
+                walk_request(walk, slots, V);
+            }
             // marks of the next node's neighbours, as soon as its record is
             // in (it was requested before the sift): every write of this
             // round is visible (barrier above) and none can happen before the
@@ -811,8 +819,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                 pf_vn[2] = pmark[a2 >= 0 ? a2 : 0];
                 pf_vn_ok = has_next;
             }
-            // corridor walk step 2 (the wait for the record covered its entry)
-            if (__ballot(walk.active) != 0ull && l < 2) walk_advance(walk, l, slab);
             // the items still to push, at most two: the dealt items other
             // than m (and d0 when it replaced the root)
             const bool skip0 = repl || m == d0;
@@ -823,7 +829,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         }
         popped_next = has_next;
         c_next = pf_c;
-        if (l < 2) walk_request(walk, slots, V);
         const unsigned long long nb = __ballot(has_next) & smask;
         const bool searching = has_req && !walking;
         rounds_here += has_req ? 1 : 0;
