@@ -41,7 +41,9 @@
 // searches, finished lane-parallel at the end) and persistent (segments take
 // searches from an atomic ticket over the active list, or one each when the
 // grid holds a segment per search, run up to `budget` rounds, park their
-// queues in HBM and take the next ticket).  In the persistent form a found
+// queues in HBM and take the next ticket).  When every agent has a slot of
+// its own (P >= N) there is no active list: segment i takes slot i if a
+// search is in flight there.  In the persistent form a found
 // search keeps its segment and walks its corridor inside the same budget,
 // two parent steps per round, parking the walk in its slot when the budget
 // runs out; the agent moves when the corridor is complete.
@@ -468,7 +470,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                 item = blockIdx.x * sp.seg + sub;
                 took_plain = true;
             }
-            if (sp.ident > 0 && item >= 0 && item < n_active && slots.agent[item] < 0) item = -1;  // slot not in flight
+            // slot = agent: nothing to do when no search is in flight there
+            if (sp.ident > 0 && item >= 0 && item < n_active && slots.agent[item] < 0) item = -1;
             if (item < 0 || item >= n_active) {
                 no_more = true;
             } else {
