@@ -125,7 +125,7 @@ def serving_main(args):
 
     loc = Locators()
     smaps = serving_maps(loc)
-    T, Q, B = args.T or 8, args.Q or 32, args.budget or 256
+    T, Q, B = args.T or 8, args.Q or 32, args.budget or 128
     steps = args.steps if args.steps_given else 300
     runs = []
     for lam in ([args.lam] if args.lam else (1000.0, 10000.0, 50000.0)):
