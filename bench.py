@@ -191,14 +191,23 @@ def crowd_bytes(g, cell, n_moving):
     return float(b), cand
 
 
+def default_hash_cell(radius):
+    """The library's default avoidance hash cell (hash_cell <= 0): just above
+    cd = 2.5 r, so neighbours within cd lie in a 3 x 3 cell stencil."""
+    return float(np.float32(np.float32(radius) * np.float32(2.5)) * np.float32(1.0 + 1.0 / 64.0))
+
+
 def crowd_main(args):
     """Crowd workload: 2048^2 with 50 convex obstacles, 65,536 agents in two
-    opposing streams, hash cell 4; one JSON line (agent-steps/s)."""
+    opposing streams, the library's default hash cell (3 x 3 stencil); one
+    JSON line (agent-steps/s)."""
     loc = Locators()
     sc = S.config4(loc)
     steps = args.steps if args.steps_given else 600
     T, Q, B = sc.T, sc.Q, args.budget or 256
-    g = make_world(sc, G.MODE_BATCHED, T, Q, B, hash_cell=4.0)
+    cell = args.hash_cell or default_hash_cell(sc.radius)
+    stencil = int(np.floor(np.float32(2.5 * sc.radius) / np.float32(cell))) + 1
+    g = make_world(sc, G.MODE_BATCHED, T, Q, B, hash_cell=cell)
     fm = []
     for f in range(args.warmup + steps):
         g.async_step(sc.dt)
@@ -211,7 +220,7 @@ def crowd_main(args):
     g.close()
     fm = np.array(fm)
     # roofline of the crowd phase: profiled frames (per-phase HIP events)
-    gp = make_world(sc, G.MODE_BATCHED, T, Q, B, profile=True, hash_cell=4.0)
+    gp = make_world(sc, G.MODE_BATCHED, T, Q, B, profile=True, hash_cell=cell)
     tot_b = tot_ms = 0.0
     cands = []
     for f in range(40):
@@ -219,7 +228,7 @@ def crowd_main(args):
         gp.async_step(sc.dt)
         gp.wait_step_end()
         if f >= 10:
-            b, cand = crowd_bytes(gp, 4.0, int((st_before == G.MOVE).sum()))
+            b, cand = crowd_bytes(gp, cell, int((st_before == G.MOVE).sum()))
             tot_b += b
             tot_ms += gp.last_frame_ms()[4]
             cands.append(cand)
@@ -227,7 +236,7 @@ def crowd_main(args):
     gbs = tot_b / (tot_ms / 1e3) / 1e9
     roof = dict(bound="hbm", achieved=round(gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(gbs / HBM_PEAK_GBS, 5), traffic=None)
-    cpu = None if args.skip_cpu else crowd_cpu_baseline(sc, loc)
+    cpu = None if args.skip_cpu else crowd_cpu_baseline(sc, loc, cell)
     out = {
         "metric": "crowd step: agent-steps/s at 65,536 agents (move + hash + avoidance + progress)",
         "value": round(sc.n * len(fm) / (fm.sum() / 1e3), 1),
@@ -243,7 +252,8 @@ def crowd_main(args):
         "data": "synthetic (seeded SplitMix64 convex obstacles and band agents)",
         "config": {
             "workload": f"{sc.name}: 50 convex obstacles (6-12 vertices, radius 20-80), {sc.n} agents r=2 v=1 in "
-                        "two opposing streams, hash cell 4 (5x5 stencil), dt 1/60",
+                        f"two opposing streams, hash cell {cell:.4g} ({2 * stencil + 1}x{2 * stencil + 1} stencil), "
+                        "dt 1/60",
             "p50_ms": round(float(np.percentile(fm, 50)), 4), "p99_ms": round(float(np.percentile(fm, 99)), 4),
             "crowd_phase_ms": round(tot_ms / 30, 4), "crowd_bytes_per_frame": round(tot_b / 30, 1),
             "neighbour_candidates_per_frame": int(np.mean(cands)),
@@ -256,7 +266,7 @@ def crowd_main(args):
     print(json.dumps(out))
 
 
-def crowd_cpu_baseline(sc, loc, seconds=12.0):
+def crowd_cpu_baseline(sc, loc, cell, seconds=12.0):
     """Single-core oracle world (exact hash avoidance) on the same scenario:
     frame 0 (the searches) untimed, then frames until ~`seconds` of CPU."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -266,7 +276,7 @@ def crowd_cpu_baseline(sc, loc, seconds=12.0):
     tri, nbr, _ = g.map_arrays()
     om = pyoracle.OracleMap(tri, nbr)
     w = pyoracle.OracleWorld(om, agents_max=sc.n, radius=sc.radius, speed=sc.speed, parallel=sc.P,
-                             threads=sc.T, queue_cap=sc.Q, heur_k=sc.K, round_budget=0, hash_cell=4.0,
+                             threads=sc.T, queue_cap=sc.Q, heur_k=sc.K, round_budget=0, hash_cell=cell,
                              use_grid_locate=1)
     for x, y in sc.starts:
         w.add_agent(x, y)
@@ -298,6 +308,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--lam", type=float, default=0.0, help="serving: one arrival rate (requests/s) only")
+    ap.add_argument("--hash-cell", type=float, default=0.0, help="crowd: avoidance hash cell (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "

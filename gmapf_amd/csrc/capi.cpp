@@ -312,7 +312,10 @@ struct gmapf_s {
             cp.rot_cs = cfg.rot_cs;
             cp.rot_sn = cfg.rot_sn;
         }
-        float cell = cfg.hash_cell > 0 ? cfg.hash_cell : cfg.agent_radius * 2.0f;
+        // default cell: just above the avoidance distance cd, so every
+        // neighbour within cd lies in the 3 x 3 cells around an agent (the
+        // margin of 1/64 is far above the rounding of the cell coordinates)
+        float cell = cfg.hash_cell > 0 ? cfg.hash_cell : cp.cd * (1.0f + 1.0f / 64.0f);
         if (!(cell > 0)) cell = 1.0f;
         dmap.hx0 = nav.bx0;
         dmap.hy0 = nav.by0;

@@ -414,15 +414,14 @@ __global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
         fr.counters[5] = 0;  // counted by the admission pass that follows
         fr.counters[6] = 0;
     }
-    // the count and this thread's entry are read together, one trip to host
-    // memory (the buffers hold one entry per agent; entries past the count
-    // are stale and unused)
-    const bool in = k < ag.n;
+    // every lane reads the count (one request per wave), then the lanes
+    // below it read their entry: a frame with few updates reads little
+    // host memory
     const int n = *fr.h_upd_n;
-    const int i = in ? fr.h_upd_ids[k] : 0;
-    const int tg = in ? fr.h_upd_targ[k] : 0;
-    const float2 xy = in ? fr.h_upd_xy[k] : make_float2(0.f, 0.f);
-    if (!in || k >= n) return;
+    if (k >= n || k >= ag.n) return;
+    const int i = fr.h_upd_ids[k];
+    const int tg = fr.h_upd_targ[k];
+    const float2 xy = fr.h_upd_xy[k];
     ag.state[i] = kSearch;
     ag.targ[i] = tg;
     ag.targ_pos[i] = xy;

@@ -50,7 +50,8 @@ typedef struct gmapf_config {
     /* build knobs, no reference counterpart */
     int step_mode;                     /* GMAPF_MODE_* */
     int round_budget;                  /* batched: search rounds per search per frame (>0) */
-    float hash_cell;                   /* avoidance hash cell size; <= 0: agent_radius * 2 */
+    float hash_cell;                   /* avoidance hash cell size; <= 0: just above 2.5 * agent_radius
+                                          (a 3 x 3 cell stencil) */
     float rot_cs, rot_sn;              /* FP32 cos/sin of the 10-degree avoidance turn;
                                           both 0: computed from 10 * pi_f / 180 */
     int device;                        /* HIP device ordinal */

@@ -42,6 +42,13 @@ def make_gpu(tri, nbr, cfg: G.Config, agents_xy, targets_xy=None):
     return g
 
 
+def default_hash_cell(radius):
+    """The library's avoidance hash cell when the config gives none: just above
+    cd = 2.5 r, in the library's FP32 arithmetic (the avoidance result does not
+    depend on the cell; the oracle uses the same one to share the grid)."""
+    return float(np.float32(np.float32(radius) * np.float32(2.5)) * np.float32(1.0 + 1.0 / 64.0))
+
+
 def make_oracle(oracle_mod, tri, nbr, cfg: G.Config, agents_xy, targets_xy=None, omap=None):
     omap = omap or oracle_mod.OracleMap(tri, nbr)
     budget = cfg.round_budget if cfg.step_mode == G.MODE_BATCHED else 0
@@ -49,7 +56,7 @@ def make_oracle(oracle_mod, tri, nbr, cfg: G.Config, agents_xy, targets_xy=None,
         omap, agents_max=cfg.agents_max_count, radius=cfg.agent_radius, speed=cfg.agent_speed,
         parallel=cfg.path_finder_parallel_agents, threads=cfg.path_finder_threads_per_agents,
         queue_cap=cfg.path_finder_queue_capacity, heur_k=cfg.path_finder_heuristic_k,
-        round_budget=budget, hash_cell=cfg.hash_cell if cfg.hash_cell > 0 else cfg.agent_radius * 2,
+        round_budget=budget, hash_cell=cfg.hash_cell if cfg.hash_cell > 0 else default_hash_cell(cfg.agent_radius),
         use_grid_locate=1)
     for x, y in agents_xy:
         assert w.add_agent(x, y) >= 0
