@@ -550,8 +550,30 @@ __global__ void k_hash_fixup(DevAgents ag, DevFrame fr) {
 
 // AgentsMover.h:96-126 over the 5x5 (generally (2k+1)^2) stencil, Jacobi, the
 // neighbour terms summed in ascending agent index.
-__global__ void k_avoid(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+constexpr int kAvoidThreads = 256;
+constexpr int kAvoidList = 16;  // neighbours an agent keeps in LDS; more are taken by rescans
+
+__device__ __forceinline__ void avoid_term(float2 p, float2 o, const CrowdParams& cp, float& sx, float& sy,
+                                           int& cnt) {
+    float dx = p.x - o.x, dy = p.y - o.y;
+    float dist = sqrtf(dx * dx + dy * dy);
+    float shift = cp.cd - dist;
+    sx += (dx / dist) * shift;
+    sy += (dy / dist) * shift;
+    ++cnt;
+}
+
+// The neighbours within cd of agent i are summed in ascending agent index.
+// One pass over the stencil puts them (in stencil order) into a per-thread
+// LDS list (entry m of thread t at m * kAvoidThreads + t: no bank
+// conflicts); the sum then takes the list's entries by ascending index.  An
+// agent with more neighbours than the list holds takes them by rescans of
+// the stencil, one per neighbour, with the same result.
+__global__ __launch_bounds__(kAvoidThreads) void k_avoid(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp) {
+    __shared__ int lj[kAvoidList * kAvoidThreads];
+    __shared__ float2 lp[kAvoidList * kAvoidThreads];
+    const int t = threadIdx.x;
+    int i = blockIdx.x * blockDim.x + t;
     if (i >= ag.n || fr.counters[5] == 0) return;
     const float2 p = ag.pos[i];
     float sx = 0.f, sy = 0.f;
@@ -561,8 +583,43 @@ __global__ void k_avoid(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp) {
         const int cx = key % map.hnx, cy = key / map.hnx;
         const int x0 = max(0, cx - map.hk), x1 = min(map.hnx - 1, cx + map.hk);
         const int y0 = max(0, cy - map.hk), y1 = min(map.hny - 1, cy + map.hk);
+        int k = 0;
+        for (int yy = y0; yy <= y1; ++yy) {
+            const int row = yy * map.hnx;
+            const int b = fr.hstart[row + x0], e = fr.hstart[row + x1 + 1];
+            for (int q = b; q < e; ++q) {
+                const int j = fr.horder[q];
+                if (j == i) continue;
+                const float2 o = ag.pos[j];
+                float dx = p.x - o.x, dy = p.y - o.y;
+                if (dx * dx + dy * dy < cp.cd2) {
+                    if (k < kAvoidList) {
+                        lj[k * kAvoidThreads + t] = j;
+                        lp[k * kAvoidThreads + t] = o;
+                    }
+                    ++k;
+                }
+            }
+        }
+        if (k <= kAvoidList) {
+            int last = -1;
+            for (int m = 0; m < k; ++m) {
+                int best = INT_MAX;
+                float2 bp = make_float2(0.f, 0.f);
+                for (int r = 0; r < k; ++r) {
+                    const int j = lj[r * kAvoidThreads + t];
+                    if (j > last && j < best) {
+                        best = j;
+                        bp = lp[r * kAvoidThreads + t];
+                    }
+                }
+                avoid_term(p, bp, cp, sx, sy, cnt);
+                last = best;
+            }
+            k = -1;  // done
+        }
         int last = -1;
-        while (true) {
+        while (k >= 0) {
             int best = INT_MAX;
             float2 bp = make_float2(0.f, 0.f);
             for (int yy = y0; yy <= y1; ++yy) {
@@ -580,12 +637,7 @@ __global__ void k_avoid(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp) {
                 }
             }
             if (best == INT_MAX) break;
-            float dx = p.x - bp.x, dy = p.y - bp.y;
-            float dist = sqrtf(dx * dx + dy * dy);
-            float shift = cp.cd - dist;
-            sx += (dx / dist) * shift;
-            sy += (dy / dist) * shift;
-            ++cnt;
+            avoid_term(p, bp, cp, sx, sy, cnt);
             last = best;
         }
     }
@@ -701,7 +753,7 @@ hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& 
         return e;
     hipLaunchKernelGGL(k_hash_scatter, dim3(nb), dim3(256), 0, st, ag, fr);
     hipLaunchKernelGGL(k_hash_fixup, dim3(nb), dim3(256), 0, st, ag, fr);
-    hipLaunchKernelGGL(k_avoid, dim3(nb), dim3(256), 0, st, map, ag, fr, cp);
+    hipLaunchKernelGGL(k_avoid, dim3(nblk(ag.n, kAvoidThreads)), dim3(kAvoidThreads), 0, st, map, ag, fr, cp);
     hipLaunchKernelGGL(k_progress, dim3(nb), dim3(256), 0, st, map, ag, fr, cp);
     return hipGetLastError();
 }

@@ -176,3 +176,26 @@ def test_one_slot_many_searches_mark_wrap(oracle, mode, budget):
             assert np.array_equal(g.corridor(a), cor), f"search {searches}"
         searches += 1
     assert found > 300
+
+
+@pytest.mark.parametrize("spread", [0.5, 20.0])
+def test_avoidance_dense_cluster_equals_oracle(oracle, spread):
+    """Agents packed within `spread` units of each other: at 0.5 each agent
+    has more neighbours within cd than the per-thread neighbour list holds,
+    so the avoidance pass takes them by rescans; at 20 the list serves.
+    Positions, states and nodes must equal the oracle bit for bit."""
+    tri, nbr = H.grid_map(3, 40, 40, cell=4.0, p=0.0)
+    r = np.random.RandomState(5)
+    n = 48
+    starts = (np.array([80.0, 80.0]) + r.uniform(-spread, spread, (n, 2))).astype(np.float32)
+    targs = (np.array([20.0, 140.0]) + r.uniform(-3, 3, (n, 2))).astype(np.float32)
+    cfg = G.Config(agents_max_count=n, path_finder_parallel_agents=n, path_finder_threads_per_agents=4,
+                   path_finder_queue_capacity=16, agent_radius=2.0, agent_speed=1.0)
+    print(f"dense cluster spread={spread} agents={n}", flush=True)
+    g = H.make_gpu(tri, nbr, cfg, starts, targs)
+    w = H.make_oracle(oracle, tri, nbr, cfg, starts, targs)
+    for f in range(40):
+        g.step(1.0 / 60)
+        w.step(1.0 / 60)
+        H.compare_worlds(g, w, f, check_corridors=(f % 10 == 0))
+    g.close()
