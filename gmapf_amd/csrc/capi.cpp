@@ -422,7 +422,6 @@ struct gmapf_s {
         dfr.hcount = dalloc<int>((size_t)dfr.ncell + 1, "hash counts");
         HIPCHK(launch_fill(dfr.hcount, 0, dfr.ncell + 1, s1));
         dfr.hstart = dalloc<int>((size_t)dfr.ncell + 1, "hash starts");
-        dfr.hscatter = dalloc<int>(Nn, "hash scatter");
         dfr.horder = dalloc<int>(Nn, "hash order");
         dfr.hblk = dalloc<int>(dfr.ncell / scan_block + 2, "hash scan partials");
         // corridors of agents, then of slots whose corridor walk is parked

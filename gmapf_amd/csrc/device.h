@@ -132,8 +132,7 @@ struct DevFrame {
     int* hkey;        // N
     int* hcount;      // ncell + 1, zero between frames
     int* hstart;      // ncell + 1
-    int* hscatter;    // N
-    int* horder;      // N
+    int* horder;      // N: agents grouped by cell (order inside a cell unspecified)
     int* hblk;        // scan partials over cells
     int ncell;
     float* stepv;                // device copy of speed * dt, made by the admission kernel

@@ -531,24 +531,14 @@ __global__ void k_hash_scatter(DevAgents ag, DevFrame fr) {
     if (i >= ag.n || fr.counters[5] == 0) return;
     const int key = fr.hkey[i];
     if (key < 0) return;
-    // fill the cell from its end down; the count returns to zero for the next frame
+    // fill the cell from its end down; the count returns to zero for the next
+    // frame.  The order inside a cell is whatever the atomics give: k_avoid
+    // takes a cell's members by ascending agent index itself.
     int p = fr.hstart[key] + atomicSub(&fr.hcount[key], 1) - 1;
-    fr.hscatter[p] = i;
+    fr.horder[p] = i;
 }
 
-// stable order inside a cell: rank = number of cell members with a smaller index
-__global__ void k_hash_fixup(DevAgents ag, DevFrame fr) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ag.n || fr.counters[5] == 0) return;
-    const int key = fr.hkey[i];
-    if (key < 0) return;
-    const int b = fr.hstart[key], e = fr.hstart[key + 1];
-    int r = 0;
-    for (int p = b; p < e; ++p) r += (fr.hscatter[p] < i) ? 1 : 0;
-    fr.horder[b + r] = i;
-}
-
-// AgentsMover.h:96-126 over the 5x5 (generally (2k+1)^2) stencil, Jacobi, the
+// AgentsMover.h:96-126 over the 3x3 (generally (2k+1)^2) stencil, Jacobi, the
 // neighbour terms summed in ascending agent index.
 constexpr int kAvoidThreads = 256;
 constexpr int kAvoidList = 16;  // neighbours an agent keeps in LDS; more are taken by rescans
@@ -752,7 +742,6 @@ hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& 
     if ((e = exscan(fr.hcount, fr.ncell, fr.hblk, fr.hstart, &fr.hstart[fr.ncell], st)) != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_hash_scatter, dim3(nb), dim3(256), 0, st, ag, fr);
-    hipLaunchKernelGGL(k_hash_fixup, dim3(nb), dim3(256), 0, st, ag, fr);
     hipLaunchKernelGGL(k_avoid, dim3(nblk(ag.n, kAvoidThreads)), dim3(kAvoidThreads), 0, st, map, ag, fr, cp);
     hipLaunchKernelGGL(k_progress, dim3(nb), dim3(256), 0, st, map, ag, fr, cp);
     return hipGetLastError();

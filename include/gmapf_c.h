@@ -140,7 +140,8 @@ int gmapf_get_stats(gmapf_handle h, long long* out8);
  * with profiling, device phases [1] admit [2] search [4] crowd ([3] unused);
  * host parts [5] enqueue [6] wait for the device [7] readback */
 int gmapf_last_frame_ms(gmapf_handle h, double* out8);
-/* spatial hash of the last frame (keys per agent, stable cell order) */
+/* spatial hash of the last frame: keys per agent, and the agents grouped by
+ * cell in ascending key order (the order inside a cell is unspecified) */
 int gmapf_get_hash(gmapf_handle h, int* keys, int* order, int* n);
 /* resolved build knobs: [0] T [1] Q [2] P [3] budget [4] search grid [5] mode [6] V */
 int gmapf_get_launch_info(gmapf_handle h, int* out8);

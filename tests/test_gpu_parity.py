@@ -110,15 +110,19 @@ def test_determinism_and_batched_equals_plain_corridors(oracle):
             assert np.array_equal(a, b)
 
 
-def test_hash_sort_is_stable_counting_sort(oracle):
+def test_hash_groups_agents_by_cell(oracle):
+    """The counting sort lists every hashed agent once, grouped by cell in
+    ascending key order (the order inside a cell is unspecified: avoidance
+    takes neighbours by agent index itself)."""
     tri, nbr, cfg, starts, targs = _world(44, 30, 30, 400, 4, 16, p=0.1)
     g = H.make_gpu(tri, nbr, cfg, starts, targs)
     for _ in range(5):
         g.step(1.0 / 60)
     keys, order = g.hash_state()
     valid = np.nonzero(keys >= 0)[0]
-    ref = valid[np.argsort(keys[valid], kind="stable")]
-    assert np.array_equal(order, ref)
+    assert len(order) == len(valid) and np.array_equal(np.sort(order), valid)
+    assert np.all(np.diff(keys[order]) >= 0)
+    assert len(np.unique(keys[valid])) > 10
 
 
 def test_device_locate_matches_host_and_oracle(oracle):
