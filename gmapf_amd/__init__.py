@@ -62,6 +62,7 @@ class _Config(C.Structure):
         ("device", C.c_int),
         ("corridor_slab", C.c_longlong),
         ("searches_per_wave", C.c_int),
+        ("search_timing", C.c_int),
     ]
 
 
@@ -95,6 +96,7 @@ class Config:
     device: int = 0
     corridor_slab: int = 0
     searches_per_wave: int = 0
+    search_timing: int = 0  # HIP events around the search launch of every k-th frame; 0: none
 
     def _to_c(self) -> _Config:
         c = _Config()

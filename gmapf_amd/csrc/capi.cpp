@@ -69,9 +69,11 @@ struct gmapf_s {
     hipStream_t s1 = nullptr, s2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_p[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    // around the search launch of every frame (recorded into the captured
-    // frame graph too): the search kernel's time, frame by frame
+    // around the search launch of a timed frame (every search_timing-th
+    // frame; recorded into the captured frame graph too).  Event nodes cost
+    // the graph launch about 10 us each, so other frames carry none
     hipEvent_t ev_s[2] = {nullptr, nullptr};
+    bool step_timed = false;  // the frame in flight records ev_s
     bool capturing = false;  // enqueue_frame runs under stream capture
     DevMap dmap{};
     DevAgents dag{};
@@ -92,11 +94,9 @@ struct gmapf_s {
     int* h_upd = nullptr;       // [0] count, then ids[N], targ[N]
     float2* h_upd_xy = nullptr;
     int upd_cap = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    // the batched frame with corridor-slab compaction before the search
-    hipGraph_t graph_c = nullptr;
-    hipGraphExec_t gexec_c = nullptr;
+    // batched frame graphs, [compaction before the search][search timed]
+    hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     bool compact_next = false;   // the slab was half full after the last frame
     long long compactions = 0, slab_in_use = 0, slab_deferred = 0;
 
@@ -130,14 +130,13 @@ struct gmapf_s {
         hipSetDevice(dev);
         if (s1) hipStreamSynchronize(s1);
         if (s2) hipStreamSynchronize(s2);
-        if (gexec) hipGraphExecDestroy(gexec);
-        if (graph) hipGraphDestroy(graph);
-        if (gexec_c) hipGraphExecDestroy(gexec_c);
-        if (graph_c) hipGraphDestroy(graph_c);
-        gexec = nullptr;
-        graph = nullptr;
-        gexec_c = nullptr;
-        graph_c = nullptr;
+        for (int c = 0; c < 2; c++)
+            for (int t = 0; t < 2; t++) {
+                if (gexec[c][t]) hipGraphExecDestroy(gexec[c][t]);
+                if (graph[c][t]) hipGraphDestroy(graph[c][t]);
+                gexec[c][t] = nullptr;
+                graph[c][t] = nullptr;
+            }
         for (void* p : dallocs) hipFree(p);
         dallocs.clear();
         if (h_step) hipHostFree(h_step);
@@ -197,6 +196,7 @@ struct gmapf_s {
             throw ApiError(GMAPF_E_ARG, "batched mode needs round_budget >= 1");
         if (cfg.searches_per_wave > 64 / T)
             throw ApiError(GMAPF_E_ARG, "searches_per_wave must be at most 64 / PathFinderThreadsPerAgents");
+        if (cfg.search_timing < 0) throw ApiError(GMAPF_E_ARG, "search_timing must be >= 0");
     }
 
     void manual_start() {
@@ -483,7 +483,7 @@ struct gmapf_s {
     }
 
     // ------------------------------------------------------------ frame
-    void enqueue_frame(bool compact) {
+    void enqueue_frame(bool compact, bool timed) {
         HIPCHK(launch_admit(dmap, dag, dsl, dfr, ident, s1));
         if (compact) HIPCHK(launch_compact(dag, dsl, dfr, s1));
         HIPCHK(hipEventRecord(ev_fork, s1));
@@ -492,9 +492,9 @@ struct gmapf_s {
         // of the graph (a plain record there only orders the capture and
         // leaves the events without timestamps)
         const unsigned evf = capturing ? hipEventRecordExternal : hipEventRecordDefault;
-        HIPCHK(hipEventRecordWithFlags(ev_s[0], s1, evf));
+        if (timed) HIPCHK(hipEventRecordWithFlags(ev_s[0], s1, evf));
         HIPCHK(launch_search(dmap, dag, dsl, dfr, sp, search_grid, persistent, s1));
-        HIPCHK(hipEventRecordWithFlags(ev_s[1], s1, evf));
+        if (timed) HIPCHK(hipEventRecordWithFlags(ev_s[1], s1, evf));
         HIPCHK(launch_crowd(dmap, dag, dfr, cp, s2));
         HIPCHK(hipEventRecord(ev_join, s2));
         HIPCHK(hipStreamWaitEvent(s1, ev_join, 0));
@@ -525,15 +525,15 @@ struct gmapf_s {
         HIPCHK(launch_export(dag, dfr, s1));
     }
 
-    // Captures one frame (with or without the slab compaction in front) into
-    // a graph and instantiates it.
-    void build_graph(bool compact) {
-        hipGraph_t& gr = compact ? graph_c : graph;
-        hipGraphExec_t& ge = compact ? gexec_c : gexec;
+    // Captures one frame (with or without the slab compaction in front, with
+    // or without events around the search) into a graph and instantiates it.
+    void build_graph(bool compact, bool timed) {
+        hipGraph_t& gr = graph[compact][timed];
+        hipGraphExec_t& ge = gexec[compact][timed];
         HIPCHK(hipStreamBeginCapture(s1, hipStreamCaptureModeThreadLocal));
         try {
             capturing = true;
-            enqueue_frame(compact);
+            enqueue_frame(compact, timed);
             capturing = false;
         } catch (...) {
             capturing = false;
@@ -572,16 +572,19 @@ struct gmapf_s {
         // last frame exported; no device round trip)
         const bool compact = compact_next;
         if (compact) compactions++;
+        const bool timed = cfg.search_timing > 0 && frames % cfg.search_timing == 0;
+        step_timed = timed;
         if (cfg.step_mode == GMAPF_MODE_BATCHED && !cfg.is_profiler && !cfg.is_debug) {
-            // both frame variants are captured on the first step, so a later
-            // frame never pays for an instantiation
-            if (!gexec) build_graph(false);
-            if (!gexec_c) build_graph(true);
-            HIPCHK(hipGraphLaunch(compact ? gexec_c : gexec, s1));
+            // every frame variant in use is captured on the first step, so a
+            // later frame never pays for an instantiation
+            for (int c = 0; c < 2; c++)
+                for (int t = 0; t < (cfg.search_timing > 0 ? 2 : 1); t++)
+                    if (!gexec[c][t]) build_graph(c != 0, t != 0);
+            HIPCHK(hipGraphLaunch(gexec[compact][timed], s1));
         } else if (cfg.is_profiler || cfg.is_debug) {
             enqueue_frame_profiled(compact);
         } else {
-            enqueue_frame(compact);
+            enqueue_frame(compact, timed);
         }
         HIPCHK(hipEventRecord(ev_done, s1));
         t_enq = Clock::now();
@@ -624,10 +627,12 @@ struct gmapf_s {
                 HIPCHK(hipEventElapsedTime(&ms, ev_p[k], ev_p[k + 1]));
                 last_ms[k + 1] = ms;
             }
-        } else {
+        } else if (step_timed) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, ev_s[0], ev_s[1]));
             last_ms[2] = ms;
+        } else {
+            last_ms[2] = -1.0;  // not a timed frame
         }
         for (int k = 0; k < 8; k++) {
             sum_ms[k] += last_ms[k];

@@ -29,12 +29,12 @@ class Locators:
         return lambda p: g.locate(p)
 
 
-def make_world(sc, mode, T, Q, budget, profile=False, hash_cell=0.0, spw=0):
+def make_world(sc, mode, T, Q, budget, profile=False, hash_cell=0.0, spw=0, timing=0):
     cfg = Config(agents_max_count=sc.n, agent_radius=sc.radius, agent_speed=sc.speed,
                  path_finder_parallel_agents=sc.P, path_finder_threads_per_agents=T,
                  path_finder_queue_capacity=Q, path_finder_heuristic_k=sc.K, step_mode=mode,
                  round_budget=max(1, budget), is_profiler=profile, hash_cell=hash_cell,
-                 searches_per_wave=spw)
+                 searches_per_wave=spw, search_timing=timing)
     g = GMAPF()
     g.init(cfg)
     sc.build(g)
@@ -54,12 +54,14 @@ def retarget(g, sc, counters):
         g.set_agent_targets(done.astype(np.int32), tg)
 
 
-def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None, spw=0):
+def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None, spw=0, timing=0):
     """Continuous re-path frames; returns timing and counter statistics.
-    `spw`: searches per search workgroup (0: 64 / T)."""
-    g = make_world(sc, mode, T, Q, budget, profile, spw=spw)
+    `spw`: searches per search workgroup (0: 64 / T).  `timing` k > 0: HIP
+    events around the search launch of every k-th frame; the search time and
+    bytes are averaged over the timed frames among the measured ones."""
+    g = make_world(sc, mode, T, Q, budget, profile, spw=spw, timing=timing)
     counters = np.array(sc.counters, copy=True)
-    frame_ms, found_d, exp_d, disc_d, corr_d, search_ms, host_ms = [], [], [], [], [], [], []
+    frame_ms, found_d, exp_d, disc_d, corr_d, search_ms, search_b, host_ms = [], [], [], [], [], [], [], []
     prev = g.stats()
     for f in range(warmup + steps):
         g.async_step(sc.dt)
@@ -68,12 +70,16 @@ def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None, s
         st = g.stats()
         if f >= warmup:
             frame_ms.append(ms[0])
-            search_ms.append(ms[2])
             host_ms.append(ms[5:8])
             found_d.append(st["found"] - prev["found"])
             exp_d.append(st["expansions"] - prev["expansions"])
             disc_d.append(st["discoveries"] - prev["discoveries"])
             corr_d.append(st["corridor_nodes"] - prev["corridor_nodes"])
+            if ms[2] >= 0:
+                # algorithmic bytes of the launch: 48 per expansion, 16 per
+                # discovery, 8 per corridor node
+                search_ms.append(ms[2])
+                search_b.append(48.0 * exp_d[-1] + 16.0 * disc_d[-1] + 8.0 * corr_d[-1])
         prev = st
         retarget(g, sc, counters)
         if log and (f == warmup - 1 or (f >= warmup and (f - warmup) % 30 == 29)):
@@ -88,10 +94,11 @@ def run_frames(sc, mode, T, Q, budget, warmup, steps, profile=False, log=None, s
         p50_ms=float(np.percentile(fm, 50)), p99_ms=float(np.percentile(fm, 99)),
         mean_ms=float(fm.mean()), found=int(np.sum(found_d)), expansions=int(np.sum(exp_d)),
         expansions_per_s=float(np.sum(exp_d) / tot_s) if tot_s > 0 else 0.0,
-        # the search kernel's HIP-event time per frame, and its algorithmic
-        # bytes (48 per expansion, 16 per discovery, 8 per corridor node)
-        search_ms=float(np.mean(search_ms)),
-        search_bytes=float(np.mean(48.0 * np.array(exp_d) + 16.0 * np.array(disc_d) + 8.0 * np.array(corr_d))),
+        # the search kernel's HIP-event time per timed frame, and the
+        # algorithmic bytes of the same launches
+        search_ms=float(np.mean(search_ms)) if search_ms else None,
+        search_bytes=float(np.mean(search_b)) if search_b else None,
+        search_frames=len(search_ms),
         host_enqueue_wait_readback_ms=[round(float(x), 4) for x in np.mean(np.array(host_ms), axis=0)],
         not_found=final["not_found"], stats=final, launch=info)
 

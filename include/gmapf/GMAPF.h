@@ -52,6 +52,7 @@ struct Config {
     int Device = 0;
     long long CorridorSlab = 0;  // corridor storage in ints; 0: sized from agents and map
     int SearchesPerWave = 0;     // searches per 64-lane search workgroup; 0: 64 / T
+    int SearchTiming = 0;        // time the search launch of every k-th frame; 0: none
 };
 
 class CuNodesMap {
@@ -97,6 +98,7 @@ class GMAPF {
         cc.device = c.Device;
         cc.corridor_slab = c.CorridorSlab;
         cc.searches_per_wave = c.SearchesPerWave;
+        cc.search_timing = c.SearchTiming;
         check(gmapf_init(h_, &cc));
     }
     void ManualStart() { check(gmapf_manual_start(h_)); }

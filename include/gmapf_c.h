@@ -64,6 +64,10 @@ typedef struct gmapf_config {
                                           <= 0: 64 / T.  Fewer than 64 / T leaves lanes idle
                                           and puts more workgroups on each SIMD, which hides
                                           memory latency when searches are too few to fill it */
+    int search_timing;                 /* HIP events around the search launch of every k-th
+                                          frame (frames 0, k, 2k, ...), read by
+                                          gmapf_last_frame_ms [2]; 0: no frame.  The two event
+                                          nodes lengthen a graph-launched frame by about 20 us */
 } gmapf_config;
 
 /* Reference CuNodesMap::Node (include/gmapf/misc/CuNodesMap.h:14-21);
@@ -138,7 +142,8 @@ int gmapf_locate_device(gmapf_handle h, int n, const float* xy, int* out); /* sa
 int gmapf_get_stats(gmapf_handle h, long long* out8);
 /* last frame, ms: [0] host wall (AsyncStep entry -> WaitStepEnd return);
  * with profiling, device phases [1] admit [2] search [4] crowd ([3] unused);
- * host parts [5] enqueue [6] wait for the device [7] readback */
+ * without, [2] is the search launch of a frame timed by search_timing (-1 for
+ * other frames); host parts [5] enqueue [6] wait for the device [7] readback */
 int gmapf_last_frame_ms(gmapf_handle h, double* out8);
 /* spatial hash of the last frame: keys per agent, and the agents grouped by
  * cell in ascending key order (the order inside a cell is unspecified) */

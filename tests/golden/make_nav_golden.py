@@ -1,5 +1,5 @@
 """Writes tests/golden/bowtie.nav: the "GMAPFNAV" fixture of the 100x100 map
-with the reference README's bow-tie wall (README.md:66-72), as this build's
+with the reference README's bow-tie wall (README.md:66-72), as the library's
 FillMapStart / FillMapObstacle / FillMapEnd produce it.  The test that reads
 it pins the byte format and the triangulation of that map.
 

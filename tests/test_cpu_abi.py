@@ -94,6 +94,17 @@ def test_searches_per_wave_bounded_by_lanes():
     g.close()
 
 
+def test_negative_search_timing_rejected():
+    g = G.GMAPF()
+    g.init(G.Config(agents_max_count=1, search_timing=-1))
+    tri, nbr = H.grid_map(1, 6, 6, cell=2.0, p=0.0)
+    g.fill_map_manually(tri, nbr)
+    with pytest.raises(G.GmapfError) as e:
+        g.manual_start()
+    assert e.value.code == -1
+    g.close()
+
+
 def test_bad_neighbour_index_rejected():
     g = G.GMAPF()
     g.init(G.Config(agents_max_count=1))

@@ -110,6 +110,36 @@ def test_determinism_and_batched_equals_plain_corridors(oracle):
             assert np.array_equal(a, b)
 
 
+def test_search_timing_marks_only_timed_frames_and_changes_no_result(oracle):
+    """search_timing k: frames 0, k, 2k, ... report the search launch's HIP
+    event time, the others -1; the event nodes in the frame graph leave every
+    result bit-identical to a run without them."""
+    tri, nbr, cfg, starts, targs = _world(36, 32, 32, 96, 4, 16)
+    cfg.step_mode = G.MODE_BATCHED
+    cfg.round_budget = 2
+    runs = []
+    for k in (0, 3):
+        cfg.search_timing = k
+        g = H.make_gpu(tri, nbr, cfg, starts, targs)
+        timed = []
+        for f in range(40):
+            g.step(1.0 / 60)
+            timed.append(g.last_frame_ms()[2])
+        xy, st = g.agents()
+        runs.append((xy.copy(), st.copy(), [g.corridor(i) for i in range(len(starts))], g.stats()))
+        g.close()
+        if k == 0:
+            assert all(t == -1.0 for t in timed)
+        else:
+            assert all((t > 0.0) == (f % k == 0) for f, t in enumerate(timed)), timed
+            assert all(t == -1.0 for f, t in enumerate(timed) if f % k)
+    assert np.array_equal(runs[0][0].view(np.uint32), runs[1][0].view(np.uint32))
+    assert np.array_equal(runs[0][1], runs[1][1])
+    for a, b in zip(runs[0][2], runs[1][2]):
+        assert (a is None) == (b is None) and (a is None or np.array_equal(a, b))
+    assert runs[0][3] == runs[1][3]
+
+
 def test_hash_groups_agents_by_cell(oracle):
     """The counting sort lists every hashed agent once, grouped by cell in
     ascending key order (the order inside a cell is unspecified: avoidance
