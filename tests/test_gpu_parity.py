@@ -86,6 +86,40 @@ def test_multi_frame_world_parity(oracle, mode, budget, P, spw):
         assert gs[k] == os_[k], (k, gs[k], os_[k])
 
 
+@pytest.mark.parametrize("mode,budget", [(G.MODE_PLAIN, 0), (G.MODE_BATCHED, 2)])
+def test_target_updates_made_during_a_step_apply_at_the_next(oracle, mode, budget):
+    """SetAgentTargPos while a step runs takes effect at the next step
+    (the reference loses it, README.md:91): the same changes made during
+    GPU step f and after oracle frame f give the same world, bit for bit,
+    including two changes of one agent (the last one wins) and changes on
+    consecutive frames."""
+    tri, nbr, cfg, starts, targs = _world(27, 36, 36, 128, 4, 16, mode, budget)
+    print(f"late target updates mode={mode} budget={budget}", flush=True)
+    g = H.make_gpu(tri, nbr, cfg, starts, targs)
+    w = H.make_oracle(oracle, tri, nbr, cfg, starts, targs)
+    dt = 1.0 / 60
+    for f in range(80):
+        changes = []
+        if f in (20, 21, 45):
+            for i in range(f % 3, 128, 5):
+                if i % 2 == 0:
+                    changes.append((i, *targs[(i * 7 + f) % 128]))
+                changes.append((i, *targs[(i * 11 + f) % 128]))
+        g.async_step(dt)
+        for i, x, y in changes:  # while the GPU step runs
+            g.set_agent_targ_pos(i, x, y)
+        g.wait_step_end()
+        w.step(dt)
+        for i, x, y in changes:  # after the oracle frame
+            assert w.set_target(i, x, y) == 0
+        H.compare_worlds(g, w, f, check_corridors=(f % 10 == 0))
+    gs, os_ = g.stats(), w.stats()
+    assert gs["found"] > 0
+    for k in ("found", "not_found", "expansions", "discoveries", "corridor_nodes", "rounds", "admitted"):
+        assert gs[k] == os_[k], (k, gs[k], os_[k])
+    g.close()
+
+
 def test_determinism_and_batched_equals_plain_corridors(oracle):
     """Same inputs twice -> identical bits; batched form gives the same
     corridors as the plain form for the same requests."""
