@@ -269,7 +269,7 @@ struct Walk {
     int n = 0;
     int idx = 0;  // slab index the next node goes to
     int term = 0;
-    int slot = 0;
+    const uint2* vis = nullptr;  // the search slot's visited table
     uint2 ent = make_uint2(0u, 0u);
 };
 
@@ -280,26 +280,27 @@ typedef __attribute__((address_space(1))) int global_int;
 
 // A step in two halves: `walk_advance` writes the node whose entry arrived
 // and moves to its parent; `walk_request` asks for the parent's entry.
+// Written with selects around the two stores, so a step is one short
+// straight run with two masked stores.
 __device__ __forceinline__ void walk_advance(Walk& w, int side, global_int* slab) {
-    if (!w.active) return;
-    if (w.n != w.term) {
-        slab[w.idx] = w.n;
-        w.idx += side == 0 ? -1 : 1;
-        w.n = (int)w.ent.y;
-    }
-    if (w.n == w.term) {
-        if (side == 1) slab[w.idx] = w.term;
-        w.active = false;
-    }
+    const bool mv = w.active && w.n != w.term;  // n is written, then its parent is next
+    if (mv) slab[w.idx] = w.n;
+    const int n = mv ? (int)w.ent.y : w.n;
+    const int idx = mv ? w.idx + (side == 0 ? -1 : 1) : w.idx;
+    const bool fin = w.active && n == w.term;  // the target side writes its terminal node
+    if (fin && side == 1) slab[idx] = w.term;
+    w.n = n;
+    w.idx = idx;
+    w.active = w.active && !fin;
 }
 
-__device__ __forceinline__ void walk_request(Walk& w, const DevSlots& slots, size_t V) {
-    if (w.active) w.ent = slots.vis[(size_t)w.slot * V + w.n];
+__device__ __forceinline__ void walk_request(Walk& w) {
+    if (w.active) w.ent = w.vis[w.n];
 }
 
-__device__ __forceinline__ void walk_step(Walk& w, int side, global_int* slab, const DevSlots& slots, size_t V) {
+__device__ __forceinline__ void walk_step(Walk& w, int side, global_int* slab) {
     walk_advance(w, side, slab);
-    walk_request(w, slots, V);
+    walk_request(w);
 }
 
 // The corridor of a found search (meet mS, mT): lane 0 of the segment takes
@@ -319,7 +320,7 @@ __device__ __forceinline__ bool start_corridor(int l, int slot, int agent, int S
     if (l < 2) {
         // a walk still pending on these lanes (their previous search in a
         // persistent segment) finishes first
-        while (walk.active) walk_step(walk, l, slab, slots, V);
+        while (walk.active) walk_step(walk, l, slab);
     }
     if (l == 0) {
         const int gS = (int)(vis[mS].x >> 1), gT = (int)(vis[mT].x >> 1);
@@ -355,7 +356,7 @@ __device__ __forceinline__ bool start_corridor(int l, int slot, int agent, int S
     if (base >= 0 && l < 2) {
         const int gS = share[1];
         walk.active = true;
-        walk.slot = slot;
+        walk.vis = vis;
         walk.n = l == 0 ? mS : mT;
         walk.term = l == 0 ? S : Tg;
         walk.idx = base + (l == 0 ? gS - 1 : gS);
@@ -514,7 +515,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                     if (l < 2) {
                         const int n = l == 0 ? pw.x : pw.z;
                         walk.active = n >= 0;
-                        walk.slot = slot;
+                        walk.vis = vis;
                         walk.n = n;
                         walk.term = l == 0 ? S : Tg;
                         walk.idx = cbase + (l == 0 ? pw.y : pw.w);
@@ -642,7 +643,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         __builtin_amdgcn_s_waitcnt(kWaitVmem0);
         if (__ballot(walk.active) != 0ull && l < 2) {
             walk_advance(walk, l, slab);
-            walk_request(walk, slots, V);
+            walk_request(walk);
         }
 
         // ---- claims and meets.  A candidate claims an unmarked node when it
@@ -808,7 +809,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             // the top of the next round covers it without waiting longer
             if (__ballot(walk.active) != 0ull && l < 2) {
                 walk_advance(walk, l, slab);
-                walk_request(walk, slots, V);
+                walk_request(walk);
             }
             // marks of the next node's neighbours, as soon as its record is
             // in (it was requested before the sift): every write of this
@@ -907,7 +908,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     // walks still pending when the last search of the workgroup ended: the
     // lanes step them together
     if (l < 2)
-        while (walk.active) walk_step(walk, l, slab, slots, V);
+        while (walk.active) walk_step(walk, l, slab);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         st_found += __shfl_xor(st_found, o, kWave);
