@@ -428,6 +428,8 @@ struct gmapf_s {
         dfr.slab_len = dalloc<int>((size_t)Nn + P + 1, "corridor lengths");
         dfr.slab_blk = dalloc<int>((Nn + P) / scan_block + 2, "corridor scan partials");
         dfr.stepv = dalloc<float>(1, "step");
+        dfr.state_prev = dalloc<int>(Nn, "exported states");
+        if (N > 0) HIPCHK(hipMemcpy(dfr.state_prev, state.data(), N * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(launch_fill(dfr.counters, 0, 16, s1));
         HIPCHK(launch_fill_u64(dfr.stats, 0, 8, s1));
         HIPCHK(launch_fill(dfr.hkey, -1, Nn, s1));
@@ -444,8 +446,10 @@ struct gmapf_s {
         HIPCHK(hipHostMalloc((void**)&h_upd_xy, (size_t)upd_cap * sizeof(float2)));
         h_upd[0] = 0;
         *h_step = 0.f;
-        // the crowd's last kernel writes only the positions it sets
+        // the crowd's last kernel writes only the positions it sets, the
+        // export pass only the states that changed
         if (N > 0) std::memcpy(h_pos_stage, pos.data(), (size_t)N * sizeof(float2));
+        if (N > 0) std::memcpy(h_state_stage, state.data(), (size_t)N * sizeof(int));
         dfr.h_step = h_step;
         dfr.h_upd_n = h_upd;
         dfr.h_upd_ids = h_upd + 1;

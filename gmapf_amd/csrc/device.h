@@ -144,7 +144,8 @@ struct DevFrame {
     const int* h_upd_targ;       // [n] target nodes
     const float2* h_upd_xy;      // [n] target positions
     float2* h_pos_out;           // [N] positions after the frame
-    int* h_state_out;            // [N] states after the frame
+    int* h_state_out;            // [N] states after the frame (written where they changed)
+    int* state_prev;             // [N] device copy of h_state_out
     unsigned long long* h_stats_out;  // [8] cumulative counters
     unsigned long long* h_slab_out;   // [2] slab top, deferred corridors
     // corridor slab compaction scratch

@@ -436,13 +436,21 @@ __global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
 // States and counters of the finished frame into pinned host memory.  The
 // positions are written there by the crowd's last kernel (the only one that
 // moves agents; it runs beside the search), so this pass after the search
-// writes 4 bytes per agent.  No kernel of the search writes host memory: a
+// writes the states of the 64-agent groups (one wave each) in which a state
+// changed since the last export, whole, so the host writes stay full lines;
+// the host buffer keeps the other groups' states (a crowd frame changes
+// few).  No kernel of the search writes host memory: a
 // store there inside the round loop would be waited for by the next round's
 // wait for all memory operations (the counter is in order), and the stores
 // would also hold up the end of the launch.
 __global__ void k_export(DevAgents ag, DevFrame fr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ag.n) fr.h_state_out[i] = ag.state[i];
+    const int s = i < ag.n ? ag.state[i] : 0;
+    const bool changed = i < ag.n && s != fr.state_prev[i];
+    if (__ballot(changed) != 0ull && i < ag.n) {
+        fr.h_state_out[i] = s;
+        if (changed) fr.state_prev[i] = s;
+    }
     if (i < 8) fr.h_stats_out[i] = fr.stats[i];
     if (i < 2) fr.h_slab_out[i] = ag.slab_top[i];  // [0] top, [1] deferred (adjacent words)
 }
