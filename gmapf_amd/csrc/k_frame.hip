@@ -144,12 +144,21 @@ __device__ int locate(const DevMap& m, float x, float y) {
     const int cx = cell_of(x, m.gx0, m.inv_cs, m.nx), cy = cell_of(y, m.gy0, m.inv_cs, m.ny);
     const size_t c = (size_t)cy * m.nx + cx;
     int best = -1;
-    for (int k = m.cell_start[c], e = m.cell_start[c + 1]; k < e; ++k) {
-        int t = m.cell_tris[k];
-        if (in_tri(m, t, x, y)) {
-            best = t;
-            break;
+    // the cell's triangles in ascending id, four at a time (their loads go
+    // out together; a short group repeats its first triangle, masked): the
+    // first that contains the point is the smallest id
+    for (int k0 = m.cell_start[c], e = m.cell_start[c + 1]; k0 < e && best < 0; k0 += 4) {
+        int tt[4];
+        bool in[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) tt[u] = m.cell_tris[k0 + u < e ? k0 + u : k0];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool r = in_tri(m, tt[u], x, y);  // every tt[u] is a valid triangle
+            in[u] = (k0 + u < e) & r;
         }
+#pragma unroll
+        for (int u = 3; u >= 0; --u) best = in[u] ? tt[u] : best;
     }
     for (int k = 0; k < m.n_global; ++k) {
         int t = m.global_tris[k];
@@ -585,17 +594,28 @@ __global__ __launch_bounds__(kAvoidThreads) void k_avoid(DevMap map, DevAgents a
         for (int yy = y0; yy <= y1; ++yy) {
             const int row = yy * map.hnx;
             const int b = fr.hstart[row + x0], e = fr.hstart[row + x1 + 1];
-            for (int q = b; q < e; ++q) {
-                const int j = fr.horder[q];
-                if (j == i) continue;
-                const float2 o = ag.pos[j];
-                float dx = p.x - o.x, dy = p.y - o.y;
-                if (dx * dx + dy * dy < cp.cd2) {
-                    if (k < kAvoidList) {
-                        lj[k * kAvoidThreads + t] = j;
-                        lp[k * kAvoidThreads + t] = o;
+            // the row's candidates four at a time, so their index and position
+            // loads go out together (a short group is padded with the agent
+            // itself, which is skipped); the list order does not matter, the
+            // sum below takes the neighbours by ascending index
+            for (int q0 = b; q0 < e; q0 += 4) {
+                int jj[4];
+                float2 oo[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) jj[u] = q0 + u < e ? fr.horder[q0 + u] : i;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) oo[u] = ag.pos[jj[u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (jj[u] == i) continue;
+                    float dx = p.x - oo[u].x, dy = p.y - oo[u].y;
+                    if (dx * dx + dy * dy < cp.cd2) {
+                        if (k < kAvoidList) {
+                            lj[k * kAvoidThreads + t] = jj[u];
+                            lp[k * kAvoidThreads + t] = oo[u];
+                        }
+                        ++k;
                     }
-                    ++k;
                 }
             }
         }
