@@ -63,6 +63,7 @@ class _Config(C.Structure):
         ("corridor_slab", C.c_longlong),
         ("searches_per_wave", C.c_int),
         ("search_timing", C.c_int),
+        ("search_workgroups", C.c_int),
     ]
 
 
@@ -97,6 +98,7 @@ class Config:
     corridor_slab: int = 0
     searches_per_wave: int = 0
     search_timing: int = 0  # HIP events around the search launch of every k-th frame; 0: none
+    search_workgroups: int = 0  # batched form: cap on the persistent search grid; 0: automatic
 
     def _to_c(self) -> _Config:
         c = _Config()

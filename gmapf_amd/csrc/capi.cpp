@@ -40,6 +40,24 @@ struct ApiError : std::runtime_error {
 
 using Clock = std::chrono::steady_clock;
 
+// The caller's current HIP device for the duration of one entry point.  The
+// reference pushes its CUDA context before device work and pops it after, so
+// an embedding host keeps its own (src/GMAPF.cu:82,127,266,308); here the
+// first device use of an entry point saves the caller's device, makes the
+// handle's current, and the scope restores the caller's on every return.
+struct DeviceScope {
+    DeviceScope** owner;
+    int prev = -1;  // the caller's device once saved
+    explicit DeviceScope(DeviceScope** o) : owner(o) { *owner = this; }
+    ~DeviceScope() {
+        *owner = nullptr;
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) hipSetDevice(prev);
+    }
+    DeviceScope(const DeviceScope&) = delete;
+    DeviceScope& operator=(const DeviceScope&) = delete;
+};
+
 }  // namespace
 
 struct gmapf_s {
@@ -100,6 +118,8 @@ struct gmapf_s {
     bool compact_next = false;   // the slab was half full after the last frame
     long long compactions = 0, slab_in_use = 0, slab_deferred = 0;
 
+    DeviceScope* scope = nullptr;  // set while an entry point runs
+
     // profiling
     Clock::time_point t_step;
     // [0] wall [1] admit [2] search [3] unused [4] crowd (device, profiler)
@@ -125,9 +145,20 @@ struct gmapf_s {
         return static_cast<T*>(p);
     }
 
+    // Saves the caller's device (once per entry point) and makes the handle's
+    // device current.
+    hipError_t enter_device() {
+        if (scope && scope->prev < 0) {
+            const hipError_t e = hipGetDevice(&scope->prev);
+            if (e != hipSuccess) return e;
+        }
+        return hipSetDevice(dev);
+    }
+    void use_device() { HIPCHK(enter_device()); }
+
     void release() {
         if (!started) return;
-        hipSetDevice(dev);
+        enter_device();
         if (s1) hipStreamSynchronize(s1);
         if (s2) hipStreamSynchronize(s2);
         for (int c = 0; c < 2; c++)
@@ -197,6 +228,7 @@ struct gmapf_s {
         if (cfg.searches_per_wave > 64 / T)
             throw ApiError(GMAPF_E_ARG, "searches_per_wave must be at most 64 / PathFinderThreadsPerAgents");
         if (cfg.search_timing < 0) throw ApiError(GMAPF_E_ARG, "search_timing must be >= 0");
+        if (cfg.search_workgroups < 0) throw ApiError(GMAPF_E_ARG, "search_workgroups must be >= 0");
     }
 
     void manual_start() {
@@ -207,7 +239,7 @@ struct gmapf_s {
         // node ids, corridor offsets and slab indices are 32-bit
         if (nav.V >= (1 << 30)) throw ApiError(GMAPF_E_ARG, "navmesh has more than 2^30 - 1 nodes");
         dev = cfg.device;
-        HIPCHK(hipSetDevice(dev));
+        use_device();
         started = true;  // from here release() frees what was allocated
         try {
             start_device();
@@ -331,7 +363,6 @@ struct gmapf_s {
         dag = DevAgents{};
         dag.n = N;
         dag.pos = dalloc<float2>(Nn, "agent positions");
-        dag.pos_tmp = dalloc<float2>(Nn, "agent positions (tmp)");
         dag.curr = dalloc<int>(Nn, "agent nodes");
         dag.targ_pos = dalloc<float2>(Nn, "agent targets");
         dag.targ = dalloc<int>(Nn, "agent target nodes");
@@ -386,6 +417,7 @@ struct gmapf_s {
         dsl.S = dalloc<int>(P, "slot start");
         dsl.Tg = dalloc<int>(P, "slot target");
         dsl.stamp = dalloc<unsigned>(P, "slot stamp");
+        dsl.clr = dalloc<unsigned>(P, "slot mark clear epoch");
         dsl.status = dalloc<int>(P, "slot status");
         dsl.mS = dalloc<int>(P, "slot meet start side");
         dsl.mT = dalloc<int>(P, "slot meet target side");
@@ -399,22 +431,19 @@ struct gmapf_s {
         HIPCHK(launch_fill(dsl.agent, -1, P, s1));
         HIPCHK(launch_fill(dsl.status, 0, P, s1));
         HIPCHK(hipMemsetAsync(dsl.stamp, 0, P * sizeof(unsigned), s1));
+        // the marks start zeroed, which is a clear of epoch 0 (stamps 0 .. 126)
+        HIPCHK(hipMemsetAsync(dsl.clr, 0, P * sizeof(unsigned), s1));
         HIPCHK(hipMemsetAsync(dsl.mark, 0, (size_t)P * dsl.VP, s1));
 
         // ---- frame scratch
         dfr = DevFrame{};
         const int scan_block = 1024;
-        dfr.flags = dalloc<int>(Nn, "flags");
-        dfr.ranks = dalloc<int>(Nn, "ranks");
-        dfr.sflags = dalloc<int>(P, "slot flags");
-        dfr.sranks = dalloc<int>(P, "slot ranks");
         dfr.blk = dalloc<int>(Nn / scan_block + 2, "scan partials");
         dfr.mblk = dalloc<int>(Nn / scan_block + 2, "moving partials");
         dfr.sblk = dalloc<int>(P / scan_block + 2, "slot scan partials");
         dfr.free_list = dalloc<int>(P, "free slots");
         dfr.adm_agent = dalloc<int>(P, "admitted agents");
         dfr.active = dalloc<int>(P, "active slots");
-        dfr.ended = dalloc<int>(P, "ended slots");
         dfr.counters = dalloc<int>(16, "counters");
         dfr.stats = dalloc<unsigned long long>(8, "stats");
         dfr.ncell = dmap.hnx * dmap.hny;
@@ -422,7 +451,11 @@ struct gmapf_s {
         dfr.hcount = dalloc<int>((size_t)dfr.ncell + 1, "hash counts");
         HIPCHK(launch_fill(dfr.hcount, 0, dfr.ncell + 1, s1));
         dfr.hstart = dalloc<int>((size_t)dfr.ncell + 1, "hash starts");
+        HIPCHK(launch_fill(dfr.hstart, 0, dfr.ncell + 1, s1));  // no agent hashed before the first mover
+        dfr.hslot = dalloc<int>(Nn, "hash arrival slots");
+        dfr.hfill = dalloc<int>(Nn, "hash fill");
         dfr.horder = dalloc<int>(Nn, "hash order");
+        dfr.hent = dalloc<float4>(Nn, "hash positions");
         dfr.hblk = dalloc<int>(dfr.ncell / scan_block + 2, "hash scan partials");
         // corridors of agents, then of slots whose corridor walk is parked
         dfr.slab_len = dalloc<int>((size_t)Nn + P + 1, "corridor lengths");
@@ -476,6 +509,7 @@ struct gmapf_s {
             if (per_cu < 1) throw ApiError(GMAPF_E_ARG, "queue capacity does not fit in LDS");
             int waves_needed = (Pe + rpw - 1) / rpw;
             search_grid = std::max(1, std::min(per_cu * cus, waves_needed));
+            if (cfg.search_workgroups > 0) search_grid = std::min(search_grid, cfg.search_workgroups);
             persistent = 1;
             sp.budget = cfg.round_budget;
         } else {
@@ -554,7 +588,7 @@ struct gmapf_s {
         require_init();
         if (step_going) return;  // GMAPF.cu:260-261
         if (!started) manual_start();
-        HIPCHK(hipSetDevice(dev));
+        use_device();
         t_step = Clock::now();
         // frame inputs go to pinned host memory that the frame's first
         // kernels read directly (no copies; the captured graph stays fixed)
@@ -598,7 +632,7 @@ struct gmapf_s {
     void wait_step_end() {
         require_init();
         if (!step_going) return;
-        HIPCHK(hipSetDevice(dev));
+        use_device();
         step_going = false;
         // poll the frame's completion event: lower wake-up latency than a
         // blocking stream synchronisation, like CUDA's default spin schedule
@@ -767,6 +801,7 @@ namespace {
 template <class F>
 int guarded(gmapf_handle h, F&& f) {
     if (!h) return GMAPF_E_ARG;
+    DeviceScope ds(&h->scope);
     try {
         f();
         h->err.clear();
@@ -818,9 +853,13 @@ int gmapf_create(gmapf_handle* out) {
 
 int gmapf_destroy(gmapf_handle h) {
     if (!h) return GMAPF_E_ARG;
-    try {
-        if (h->step_going) h->wait_step_end();
-    } catch (...) {
+    {
+        DeviceScope ds(&h->scope);
+        try {
+            if (h->step_going) h->wait_step_end();
+        } catch (...) {
+        }
+        h->release();
     }
     delete h;
     return GMAPF_OK;
@@ -962,6 +1001,7 @@ int gmapf_get_agents_soa(gmapf_handle h, float* xy, int* st, int* node) {
         if (node) {
             if (h->started && N > 0) {
                 if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
+                h->use_device();
                 HIPCHK(hipMemcpy(node, h->dag.curr, (size_t)N * sizeof(int), hipMemcpyDeviceToHost));
             } else if (N > 0) {
                 std::memcpy(node, h->curr.data(), (size_t)N * sizeof(int));
@@ -979,6 +1019,7 @@ int gmapf_get_agent_corridor(gmapf_handle h, int id, int* buf, int cap, int* len
             return;
         }
         if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
+        h->use_device();
         int L = 0;
         long long off = 0;
         int* slab = nullptr;
@@ -1005,6 +1046,7 @@ int gmapf_get_agent_steps(gmapf_handle h, int* step_idx, int* step_node, float* 
         if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
         int N = h->n_agents;
         if (N == 0) return;
+        h->use_device();
         if (step_idx) HIPCHK(hipMemcpy(step_idx, h->dag.step_idx, N * sizeof(int), hipMemcpyDeviceToHost));
         if (step_node) HIPCHK(hipMemcpy(step_node, h->dag.step_node, N * sizeof(int), hipMemcpyDeviceToHost));
         if (step_xy) HIPCHK(hipMemcpy(step_xy, h->dag.step_pos, N * sizeof(float2), hipMemcpyDeviceToHost));
@@ -1113,6 +1155,7 @@ int gmapf_locate_device(gmapf_handle h, int n, const float* xy, int* out) {
         if (!h->started) throw ApiError(GMAPF_E_STATE, "not started");
         if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
         if (n <= 0) return;
+        h->use_device();
         float2* dp = nullptr;
         int* dout = nullptr;
         HIPCHK(hipMalloc(&dp, n * sizeof(float2)));
@@ -1148,7 +1191,9 @@ int gmapf_get_hash(gmapf_handle h, int* keys, int* order, int* n) {
         if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
         int N = h->n_agents;
         int total = 0;
+        h->use_device();
         HIPCHK(hipMemcpy(&total, h->dfr.hstart + h->dfr.ncell, sizeof(int), hipMemcpyDeviceToHost));
+        if (total < 0 || total > N) throw ApiError(GMAPF_E_INTERNAL, "hash total out of range");
         if (keys && N) HIPCHK(hipMemcpy(keys, h->dfr.hkey, N * sizeof(int), hipMemcpyDeviceToHost));
         if (order && total > 0) HIPCHK(hipMemcpy(order, h->dfr.horder, total * sizeof(int), hipMemcpyDeviceToHost));
         if (n) *n = total;
@@ -1160,7 +1205,7 @@ int gmapf_get_slab_info(gmapf_handle h, long long* out4) {
         h->require_init();
         if (!out4) throw ApiError(GMAPF_E_ARG, "null output");
         out4[0] = h->started ? h->dag.slab_cap : 0;
-        out4[1] = h->slab_in_use;
+        out4[1] = std::min(h->slab_in_use, h->started ? h->dag.slab_cap : 0LL);
         out4[2] = h->compactions;
         out4[3] = h->slab_deferred;
     });

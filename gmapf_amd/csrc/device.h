@@ -29,8 +29,9 @@
 //   search that claimed the node: the probe a search makes for every
 //   neighbour reads one byte (64 nodes per cache line); a node belongs to
 //   the current search only when its m matches, so marks are cleared only
-//   when m wraps (once every 127 searches of a slot), and tables are never
-//   cleared per search (replaces PathFinder.h:119-136).
+//   once per epoch of 127 stamps (by the epoch's first search to seed in
+//   the slot, recorded in clr), and tables are never cleared per search
+//   (replaces PathFinder.h:119-136).
 //   vis[slot * V + node] uint2 {G << 1 | dir, parent}: read for the node a
 //   lane pops and by the corridor walk, i.e. only for nodes of the current
 //   search, so it needs no tag.
@@ -65,7 +66,6 @@ struct DevMap {
 struct DevAgents {
     int n;
     float2* pos;
-    float2* pos_tmp;
     int* curr;
     float2* targ_pos;
     int* targ;
@@ -93,6 +93,7 @@ struct DevSlots {
     int* S;
     int* Tg;
     unsigned* stamp;
+    unsigned* clr;  // epoch (stamp / 127) of the slot's last mark clear
     int* status;    // 0 running, 1 found, 2 exhausted, 3 fresh (needs seeding),
                     // 4 found, corridor deferred (needs slab space),
                     // 5 found, corridor walk parked (batched form)
@@ -110,17 +111,12 @@ struct DevSlots {
 
 // Per-frame scratch / counters (device).
 struct DevFrame {
-    int* flags;      // N   request flags
-    int* ranks;      // N
-    int* sflags;     // P   slot flags
-    int* sranks;     // P
     int* blk;        // per-block request counts, then their exclusive prefixes
     int* mblk;       // per-block moving-agent counts
     int* sblk;       // per-block free-slot counts, then their exclusive prefixes
     int* free_list;  // P
     int* adm_agent;  // P: agent of the k-th admission this frame
     int* active;     // P: active slot list for the search kernel
-    int* ended;      // P: slots that ended this frame
     int* counters;   // [0] n_requests [1] n_active [2] work ticket [3] n_ended
                      // [4] n_free [5] moving count [6] n_admitted
                      // [7] blocks done in the admission count pass
@@ -129,10 +125,13 @@ struct DevFrame {
     unsigned long long* stats;  // [0] found [1] fail [2] expansions [3] discoveries
                                 // [4] corridor nodes [5] rounds [6] admitted
     // spatial hash
-    int* hkey;        // N
+    int* hkey;        // N: cell key of the moved position, -1 for a NaN position
+    int* hslot;       // N: arrival slot of the agent in its cell (histogram atomic)
     int* hcount;      // ncell + 1, zero between frames
     int* hstart;      // ncell + 1
-    int* horder;      // N: agents grouped by cell (order inside a cell unspecified)
+    int* hfill;       // N: agents grouped by cell, arrival order inside a cell
+    int* horder;      // N: agents grouped by cell, ascending index inside a cell
+    float4* hent;     // N: {x, y, bits of the agent index, 0} in horder's order
     int* hblk;        // scan partials over cells
     int ncell;
     float* stepv;                // device copy of speed * dt, made by the admission kernel

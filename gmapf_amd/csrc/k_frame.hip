@@ -47,9 +47,10 @@ __device__ __forceinline__ int block_exscan(int v, int* sh, int* total) {
     return wpre + x - v;
 }
 
-// per-block sums of vals[0, n)
-__global__ __launch_bounds__(kScanThreads) void k_block_sum(const int* vals, int n, int* blk) {
+// per-block sums of vals[0, n); nothing when *gate is 0 (gate may be null)
+__global__ __launch_bounds__(kScanThreads) void k_block_sum(const int* vals, int n, int* blk, const int* gate) {
     __shared__ int sh[4];
+    if (gate && *gate == 0) return;
     const long long base = (long long)blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
     int s = 0;
 #pragma unroll
@@ -59,11 +60,13 @@ __global__ __launch_bounds__(kScanThreads) void k_block_sum(const int* vals, int
     if (threadIdx.x == 0) blk[blockIdx.x] = t;
 }
 
-// exclusive scan of vals into out; *total = sum of all
+// exclusive scan of vals into out; *total = sum of all.  clear (null or
+// vals itself): zero the values once read.  Nothing when *gate is 0.
 __global__ __launch_bounds__(kScanThreads) void k_block_scan(const int* vals, int n, const int* blk,
-                                                             int* out, int* total) {
+                                                             int* out, int* total, int* clear, const int* gate) {
     __shared__ int sh[4];
     __shared__ int sh2[4];
+    if (gate && *gate == 0) return;
     int o = 0;
     for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) o += blk[b];
     int off = block_reduce_sum(o, sh);
@@ -79,6 +82,11 @@ __global__ __launch_bounds__(kScanThreads) void k_block_scan(const int* vals, in
         v[k] = (base + k < n) ? vals[base + k] : 0;
         s += v[k];
     }
+    if (clear) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+            if (base + k < n) clear[base + k] = 0;
+    }
     int btot = 0;
     int pre = block_exscan(s, sh2, &btot);
     int run = off + pre;
@@ -90,12 +98,13 @@ __global__ __launch_bounds__(kScanThreads) void k_block_scan(const int* vals, in
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0 && total) *total = off + btot;
 }
 
-hipError_t exscan(const int* vals, int n, int* blk, int* out, int* total, hipStream_t st) {
+hipError_t exscan(const int* vals, int n, int* blk, int* out, int* total, hipStream_t st, int* clear = nullptr,
+                  const int* gate = nullptr) {
     if (n <= 0) return hipSuccess;
     int nb = (n + kScanBlock - 1) / kScanBlock;
-    hipLaunchKernelGGL(k_block_sum, dim3(nb), dim3(kScanThreads), 0, st, vals, n, blk);
+    hipLaunchKernelGGL(k_block_sum, dim3(nb), dim3(kScanThreads), 0, st, vals, n, blk, gate);
     hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(kScanThreads), 0, st, vals, n, (const int*)blk,
-                       out, total);
+                       out, total, clear, gate);
     return hipGetLastError();
 }
 
@@ -490,7 +499,13 @@ __global__ void k_slab_copy(DevAgents ag, DevSlots sl, DevFrame fr) {
     for (int a = w; a < ag.n + sl.P; a += nw) {
         const int off = fr.slab_len[a];
         const int len = fr.slab_len[a + 1] - off;
-        if (len == 0) continue;
+        if (len == 0) {
+            // a released corridor (the agent is not moving): the agent has
+            // none from now on, so no reader follows its old offset into the
+            // new buffer
+            if (lane == 0 && a < ag.n) ag.path_len[a] = 0;
+            continue;
+        }
         const long long from = a < ag.n ? ag.path_off[a] : (long long)sl.corr[a - ag.n].x;
         for (int k = lane; k < len; k += 64) dst[off + k] = src[from + k];
         if (lane == 0) {
@@ -512,9 +527,19 @@ __global__ void k_slab_finish(DevAgents ag, DevSlots sl, DevFrame fr) {
 }
 
 // ------------------------------------------------------------- crowd
+// Spatial hash = stable counting sort of the agents by cell key (the order
+// std::stable_sort by key gives):
+//   k_move_key     move, key of the moved position, histogram; the atomic's
+//                  return value (the agent's arrival slot in its cell) is kept
+//   scan           exclusive prefix of the counts -> cell starts; the counts
+//                  are zeroed as they are read, ready for the next frame
+//   k_hash_place   each agent to start + arrival slot (cell groups complete,
+//                  order inside a cell is arrival order)
+//   k_hash_rank    each agent's rank in its cell = members with a smaller
+//                  index; the final order and a copy of the positions in that
+//                  order (the avoidance pass reads only the copy)
 // AgentsMover.h:57-70 (move), then the agent's hash cell from its moved
-// position, counted into hcount (all zero at the start of a frame: the
-// scatter counts every cell back down)
+// position, counted into hcount (all zero at the start of a frame)
 __global__ void k_move_key(DevMap map, DevAgents ag, DevFrame fr) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ag.n || fr.counters[5] == 0) return;
@@ -538,25 +563,37 @@ __global__ void k_move_key(DevMap map, DevAgents ag, DevFrame fr) {
         int cx = cell_of(p.x, map.hx0, map.hinv, map.hnx);
         int cy = cell_of(p.y, map.hy0, map.hinv, map.hny);
         key = cy * map.hnx + cx;
-        atomicAdd(&fr.hcount[key], 1);
+        fr.hslot[i] = atomicAdd(&fr.hcount[key], 1);
     }
     fr.hkey[i] = key;
 }
 
-__global__ void k_hash_scatter(DevAgents ag, DevFrame fr) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_hash_place(DevAgents ag, DevFrame fr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ag.n || fr.counters[5] == 0) return;
     const int key = fr.hkey[i];
     if (key < 0) return;
-    // fill the cell from its end down; the count returns to zero for the next
-    // frame.  The order inside a cell is whatever the atomics give: k_avoid
-    // takes a cell's members by ascending agent index itself.
-    int p = fr.hstart[key] + atomicSub(&fr.hcount[key], 1) - 1;
-    fr.horder[p] = i;
+    fr.hfill[fr.hstart[key] + fr.hslot[i]] = i;
+}
+
+__global__ void k_hash_rank(DevAgents ag, DevFrame fr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ag.n || fr.counters[5] == 0) return;
+    const int key = fr.hkey[i];
+    if (key < 0) return;
+    const int b = fr.hstart[key], e = fr.hstart[key + 1];
+    int r = 0;
+    for (int q = b; q < e; ++q) r += fr.hfill[q] < i ? 1 : 0;
+    const float2 p = ag.pos[i];
+    fr.horder[b + r] = i;
+    fr.hent[b + r] = make_float4(p.x, p.y, __int_as_float(i), 0.f);
 }
 
 // AgentsMover.h:96-126 over the 3x3 (generally (2k+1)^2) stencil, Jacobi, the
-// neighbour terms summed in ascending agent index.
+// neighbour terms summed in ascending agent index, then AgentsMover.h:137-176
+// (path progress) for the moving set.  Neighbours are read from the hash's
+// position copy (hent), never from pos, so the pass writes each agent's new
+// position in place and the sum still sees only post-move positions.
 constexpr int kAvoidThreads = 256;
 constexpr int kAvoidList = 16;  // neighbours an agent keeps in LDS; more are taken by rescans
 
@@ -571,16 +608,17 @@ __device__ __forceinline__ void avoid_term(float2 p, float2 o, const CrowdParams
 }
 
 // The neighbours within cd of agent i are summed in ascending agent index.
-// One pass over the stencil puts them (in stencil order) into a per-thread
-// LDS list (entry m of thread t at m * kAvoidThreads + t: no bank
-// conflicts); the sum then takes the list's entries by ascending index.  An
-// agent with more neighbours than the list holds takes them by rescans of
-// the stencil, one per neighbour, with the same result.
-__global__ __launch_bounds__(kAvoidThreads) void k_avoid(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp) {
+// One pass over the stencil puts them into a per-thread LDS list (entry m of
+// thread t at m * kAvoidThreads + t: no bank conflicts); the sum then takes
+// the list's entries by ascending index.  An agent with more neighbours than
+// the list holds takes them by rescans of the stencil, one per neighbour,
+// with the same result.
+__global__ __launch_bounds__(kAvoidThreads) void k_avoid_progress(DevMap map, DevAgents ag, DevFrame fr,
+                                                                  CrowdParams cp) {
     __shared__ int lj[kAvoidList * kAvoidThreads];
     __shared__ float2 lp[kAvoidList * kAvoidThreads];
     const int t = threadIdx.x;
-    int i = blockIdx.x * blockDim.x + t;
+    const int i = blockIdx.x * blockDim.x + t;
     if (i >= ag.n || fr.counters[5] == 0) return;
     const float2 p = ag.pos[i];
     float sx = 0.f, sy = 0.f;
@@ -590,29 +628,28 @@ __global__ __launch_bounds__(kAvoidThreads) void k_avoid(DevMap map, DevAgents a
         const int cx = key % map.hnx, cy = key / map.hnx;
         const int x0 = max(0, cx - map.hk), x1 = min(map.hnx - 1, cx + map.hk);
         const int y0 = max(0, cy - map.hk), y1 = min(map.hny - 1, cy + map.hk);
+        const float4 self = make_float4(p.x, p.y, __int_as_float(i), 0.f);
         int k = 0;
         for (int yy = y0; yy <= y1; ++yy) {
             const int row = yy * map.hnx;
             const int b = fr.hstart[row + x0], e = fr.hstart[row + x1 + 1];
-            // the row's candidates four at a time, so their index and position
-            // loads go out together (a short group is padded with the agent
-            // itself, which is skipped); the list order does not matter, the
-            // sum below takes the neighbours by ascending index
+            // the row's candidates four at a time, so their loads go out
+            // together (a short group is padded with the agent itself, which
+            // is skipped); the list order does not matter, the sum below
+            // takes the neighbours by ascending index
             for (int q0 = b; q0 < e; q0 += 4) {
-                int jj[4];
-                float2 oo[4];
+                float4 ee[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) jj[u] = q0 + u < e ? fr.horder[q0 + u] : i;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) oo[u] = ag.pos[jj[u]];
+                for (int u = 0; u < 4; ++u) ee[u] = q0 + u < e ? fr.hent[q0 + u] : self;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    if (jj[u] == i) continue;
-                    float dx = p.x - oo[u].x, dy = p.y - oo[u].y;
+                    const int j = __float_as_int(ee[u].z);
+                    if (j == i) continue;
+                    float dx = p.x - ee[u].x, dy = p.y - ee[u].y;
                     if (dx * dx + dy * dy < cp.cd2) {
                         if (k < kAvoidList) {
-                            lj[k * kAvoidThreads + t] = jj[u];
-                            lp[k * kAvoidThreads + t] = oo[u];
+                            lj[k * kAvoidThreads + t] = j;
+                            lp[k * kAvoidThreads + t] = make_float2(ee[u].x, ee[u].y);
                         }
                         ++k;
                     }
@@ -644,13 +681,13 @@ __global__ __launch_bounds__(kAvoidThreads) void k_avoid(DevMap map, DevAgents a
                 const int row = yy * map.hnx;
                 const int b = fr.hstart[row + x0], e = fr.hstart[row + x1 + 1];
                 for (int q = b; q < e; ++q) {
-                    const int j = fr.horder[q];
+                    const float4 o = fr.hent[q];
+                    const int j = __float_as_int(o.z);
                     if (j <= last || j >= best || j == i) continue;
-                    const float2 o = ag.pos[j];
                     float dx = p.x - o.x, dy = p.y - o.y;
                     if (dx * dx + dy * dy < cp.cd2) {
                         best = j;
-                        bp = o;
+                        bp = make_float2(o.x, o.y);
                     }
                 }
             }
@@ -668,16 +705,9 @@ __global__ __launch_bounds__(kAvoidThreads) void k_avoid(DevMap map, DevAgents a
         const float nx = p.x + rx, ny = p.y + ry;
         if (locate(map, nx, ny) >= 0) out = make_float2(nx, ny);
     }
-    ag.pos_tmp[i] = out;
-}
-
-// AgentsMover.h:137-176 (after the Jacobi write-back)
-__global__ void k_progress(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ag.n || fr.counters[5] == 0) return;
-    const float2 p = ag.pos_tmp[i];
-    ag.pos[i] = p;
-    fr.h_pos_out[i] = p;
+    ag.pos[i] = out;
+    fr.h_pos_out[i] = out;
+    // AgentsMover.h:137-176 (after the Jacobi write-back)
     if (!ag.in_m[i]) return;
     const int* path = *ag.slab_now + ag.path_off[i];
     int idx = ag.step_idx[i];
@@ -689,16 +719,16 @@ __global__ void k_progress(DevMap map, DevAgents ag, DevFrame fr, CrowdParams cp
         return;
     }
     const int node = ag.step_node[i];
-    if (!in_tri(map, node, p.x, p.y)) return;
+    if (!in_tri(map, node, out.x, out.y)) return;
     ag.curr[i] = node;
     const int count = ag.path_len[i];
     if (idx < count) idx += 1;
     ag.step_idx[i] = idx;
     if (idx == count) {
-        const float2 t = ag.targ_pos[i];
-        float dx = t.x - p.x, dy = t.y - p.y;
+        const float2 tp = ag.targ_pos[i];
+        float dx = tp.x - out.x, dy = tp.y - out.y;
         if (sqrtf(dx * dx + dy * dy) > FLT_EPSILON + cp.slack)
-            ag.step_pos[i] = t;
+            ag.step_pos[i] = tp;
         else
             ag.state[i] = kIdle;
     } else {
@@ -767,11 +797,14 @@ hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& 
     hipError_t e;
     const int nb = nblk(ag.n, 256);
     hipLaunchKernelGGL(k_move_key, dim3(nb), dim3(256), 0, st, map, ag, fr);
-    if ((e = exscan(fr.hcount, fr.ncell, fr.hblk, fr.hstart, &fr.hstart[fr.ncell], st)) != hipSuccess)
+    // the scan reads and zeroes the counts; nothing in a frame with no mover
+    if ((e = exscan(fr.hcount, fr.ncell, fr.hblk, fr.hstart, &fr.hstart[fr.ncell], st, fr.hcount,
+                    &fr.counters[5])) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_hash_scatter, dim3(nb), dim3(256), 0, st, ag, fr);
-    hipLaunchKernelGGL(k_avoid, dim3(nblk(ag.n, kAvoidThreads)), dim3(kAvoidThreads), 0, st, map, ag, fr, cp);
-    hipLaunchKernelGGL(k_progress, dim3(nb), dim3(256), 0, st, map, ag, fr, cp);
+    hipLaunchKernelGGL(k_hash_place, dim3(nb), dim3(256), 0, st, ag, fr);
+    hipLaunchKernelGGL(k_hash_rank, dim3(nb), dim3(256), 0, st, ag, fr);
+    hipLaunchKernelGGL(k_avoid_progress, dim3(nblk(ag.n, kAvoidThreads)), dim3(kAvoidThreads), 0, st, map, ag, fr,
+                       cp);
     return hipGetLastError();
 }
 

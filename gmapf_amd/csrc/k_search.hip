@@ -327,6 +327,15 @@ __device__ __forceinline__ bool start_corridor(int l, int slot, int agent, int S
         const int len = gS + gT + 1;
         const unsigned long long o = atomicAdd(ag.slab_top, (unsigned long long)len);
         const bool ok = o + (unsigned long long)len <= (unsigned long long)ag.slab_cap;
+        // A request that does not fit takes its add back when no allocation
+        // came after it (the top still reads o + len), so a later, shorter
+        // request can use the space left.  The compare-and-swap succeeds only
+        // when the add was the last one, so no range is handed out twice;
+        // otherwise the top stays past the end until the next compaction.
+        // (A compare-and-swap loop for every allocation would be exact, but
+        // under contention it completes one allocation per L2 round trip:
+        // 40x slower at 16,384 searches.)
+        if (!ok) atomicCAS(ag.slab_top, o + (unsigned long long)len, o);
         share[0] = ok ? (int)o : -1;
         share[1] = gS;
         share[2] = len;
@@ -460,118 +469,133 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         // test when every segment is busy; rare paths of the round are
         // skipped the same way, so a round pays no lane-mask bookkeeping for
         // them)
-        if (__ballot(!has_req && !no_more) != 0ull && !has_req && !no_more) {
-            int item = -1;
-            if (persistent && !one_each) {
-                if (l == 0) item = atomicAdd(&fr.counters[2], 1);
-                item = __shfl(item, sub * T, kWave);
-            } else if (!took_plain) {
-                // every search has a segment of its own: segment i takes
-                // item i, once (the plain form always does)
-                item = blockIdx.x * sp.seg + sub;
-                took_plain = true;
-            }
-            // slot = agent: nothing to do when no search is in flight there
-            if (sp.ident > 0 && item >= 0 && item < n_active && slots.agent[item] < 0) item = -1;
-            if (item < 0 || item >= n_active) {
-                no_more = true;
-            } else {
-                slot = sp.ident > 0 ? item : fr.active[(int)(((long long)item * stride) % n_active)];
-                agent = slots.agent[slot];
-                S = slots.S[slot];
-                Tg = slots.Tg[slot];
-                mval = slots.stamp[slot] % 127u + 1u;
-                status = slots.status[slot];
-                cS = map.cen[S];
-                cT = map.cen[Tg];
-                rounds_here = 0;
-                srch_here = 0;
-                walking = false;
-                exp_here = 0;
-                disc_here = 0;
-                vis = slots.vis + (size_t)slot * V;
-                mark = slots.mark + (size_t)slot * slots.VP;
-                pf_c = -1;
-                pf_vn_ok = false;
-                need_pop = status == 3;
-                popped_next = false;
-                has_req = true;
-                if (status == 4) {
-                    // found earlier, corridor deferred for lack of slab space
-                    const bool hold = sp.budget > 0;
-                    const bool ok = start_corridor(l, slot, agent, S, Tg, slots.mS[slot], slots.mT[slot], vis, slab,
-                                                   ag, slots, V, &wshare[3 * sub], walk, hold, cbase, clen, st_found,
-                                                   st_corr);
-                    walking = hold && ok;
-                    has_req = walking;
-                    status = 0;
-                } else if (status == 5) {
-                    // batched form: a corridor walk parked at the end of the
-                    // last launch resumes
-                    const int4 pw = slots.walk[slot];
-                    const int2 cr = slots.corr[slot];
-                    cbase = cr.x;
-                    clen = cr.y;
-                    if (l < 2) {
-                        const int n = l == 0 ? pw.x : pw.z;
-                        walk.active = n >= 0;
-                        walk.vis = vis;
-                        walk.n = n;
-                        walk.term = l == 0 ? S : Tg;
-                        walk.idx = cbase + (l == 0 ? pw.y : pw.w);
-                        if (walk.active) walk.ent = vis[n];
-                    }
-                    walking = true;
-                    status = 0;
-                } else if (status == 3) {
-                    if (mval == 1u) {
-                        // generation wrapped (every 127 searches of the slot):
-                        // zero the slot's marks before the seeds
-                        uint4* m4 = reinterpret_cast<uint4*>(mark);
-                        for (long long w = l; w < slots.VP / 16; w += T) m4[w] = make_uint4(0u, 0u, 0u, 0u);
-                        wave_sync();
-                    }
-                    // PathFinder.h:168-189
-                    const float F0 = dist2(cS, cT) * sp.K;
-                    cnt = 0;
-                    for (int i = 0; i < Q; ++i) q[i * kWave + L] = kKeyNone;
-                    if (l == 0) {
-                        q[L] = make_key(F0, S);
-                        cnt = 1;
-                        vis[S] = make_uint2(1u, 0xffffffffu);  // G 0, dir 1
-                        mark[S] = (unsigned char)((mval << 1) | 1u);
-                    }
-                    if (l == 1) {
-                        q[L] = make_key(F0, Tg);
-                        cnt = 1;
-                        vis[Tg] = make_uint2(0u, 0xffffffffu);  // G 0, dir 0
-                        mark[Tg] = (unsigned char)(mval << 1);
-                    }
-                    status = 0;
+        if (__ballot(!has_req && !no_more) != 0ull) {
+            if (!has_req && !no_more) {
+                int item = -1;
+                if (persistent && !one_each) {
+                    if (l == 0) item = atomicAdd(&fr.counters[2], 1);
+                    item = __shfl(item, sub * T, kWave);
+                } else if (!took_plain) {
+                    // every search has a segment of its own: segment i takes
+                    // item i, once (the plain form always does)
+                    item = blockIdx.x * sp.seg + sub;
+                    took_plain = true;
+                }
+                // slot = agent: nothing to do when no search is in flight there
+                if (sp.ident > 0 && item >= 0 && item < n_active && slots.agent[item] < 0) item = -1;
+                if (item < 0 || item >= n_active) {
+                    no_more = true;
                 } else {
-                    // parked: [node to expand next or kKeyNone, heap items]
-                    // (at most Q - 1 heap items).  Loads go in groups of 8
-                    // with clamped in-bounds indices and the count applied
-                    // afterwards, so a group waits for memory once
-                    const size_t qb = ((size_t)slot * T + l);
-                    cnt = slots.qcnt[qb];
-                    const unsigned long long* src = slots.qsave + qb * (size_t)Q;
-                    const unsigned long long k0 = src[0];
-                    popped_next = k0 != kKeyNone;
-                    c_next = key_node(k0);
-                    for (int i0 = 0; i0 < Q - 1; i0 += 8) {
-                        unsigned long long x[8];
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) x[j] = src[1 + min(i0 + j, Q - 2)];
-#pragma unroll
-                        for (int j = 0; j < 8; ++j)
-                            if (i0 + j < Q - 1) q[(i0 + j) * kWave + L] = i0 + j < cnt ? x[j] : kKeyNone;
+                    slot = sp.ident > 0 ? item : fr.active[(int)(((long long)item * stride) % n_active)];
+                    agent = slots.agent[slot];
+                    S = slots.S[slot];
+                    Tg = slots.Tg[slot];
+                    const unsigned stamp = slots.stamp[slot];
+                    mval = stamp % 127u + 1u;
+                    status = slots.status[slot];
+                    cS = map.cen[S];
+                    cT = map.cen[Tg];
+                    rounds_here = 0;
+                    srch_here = 0;
+                    walking = false;
+                    exp_here = 0;
+                    disc_here = 0;
+                    vis = slots.vis + (size_t)slot * V;
+                    mark = slots.mark + (size_t)slot * slots.VP;
+                    pf_c = -1;
+                    pf_vn_ok = false;
+                    need_pop = status == 3;
+                    popped_next = false;
+                    has_req = true;
+                    if (status == 4) {
+                        // found earlier, corridor deferred for lack of slab space
+                        const bool hold = sp.budget > 0;
+                        const bool ok = start_corridor(l, slot, agent, S, Tg, slots.mS[slot], slots.mT[slot], vis, slab,
+                                                       ag, slots, V, &wshare[3 * sub], walk, hold, cbase, clen, st_found,
+                                                       st_corr);
+                        walking = hold && ok;
+                        has_req = walking;
+                        status = 0;
+                    } else if (status == 5) {
+                        // batched form: a corridor walk parked at the end of the
+                        // last launch resumes
+                        const int4 pw = slots.walk[slot];
+                        const int2 cr = slots.corr[slot];
+                        cbase = cr.x;
+                        clen = cr.y;
+                        if (l < 2) {
+                            const int n = l == 0 ? pw.x : pw.z;
+                            walk.active = n >= 0;
+                            walk.vis = vis;
+                            walk.n = n;
+                            walk.term = l == 0 ? S : Tg;
+                            walk.idx = cbase + (l == 0 ? pw.y : pw.w);
+                            if (walk.active) walk.ent = vis[n];
+                        }
+                        walking = true;
+                        status = 0;
+                    } else if (status == 3) {
+                        // Stamps 127e .. 127e + 126 (epoch e) use generations
+                        // 1 .. 127 once each.  The first search of an epoch to
+                        // seed in this slot zeroes the slot's marks before its
+                        // seeds, so only marks of the current epoch remain; a
+                        // search admitted and dropped before it seeded (its
+                        // agent re-targeted) cannot skip the clear.
+                        const unsigned epoch = stamp / 127u;
+                        if (slots.clr[slot] != epoch) {
+                            uint4* m4 = reinterpret_cast<uint4*>(mark);
+                            for (long long w = l; w < slots.VP / 16; w += T) m4[w] = make_uint4(0u, 0u, 0u, 0u);
+                            wave_sync();
+                            if (l == 0) slots.clr[slot] = epoch;
+                        }
+                        // PathFinder.h:168-189
+                        const float F0 = dist2(cS, cT) * sp.K;
+                        cnt = 0;
+                        for (int i = 0; i < Q; ++i) q[i * kWave + L] = kKeyNone;
+                        if (l == 0) {
+                            q[L] = make_key(F0, S);
+                            cnt = 1;
+                            vis[S] = make_uint2(1u, 0xffffffffu);  // G 0, dir 1
+                            mark[S] = (unsigned char)((mval << 1) | 1u);
+                        }
+                        if (l == 1) {
+                            q[L] = make_key(F0, Tg);
+                            cnt = 1;
+                            vis[Tg] = make_uint2(0u, 0xffffffffu);  // G 0, dir 0
+                            mark[Tg] = (unsigned char)(mval << 1);
+                        }
+                        status = 0;
+                    } else {
+                        // parked: [node to expand next or kKeyNone, heap items]
+                        // (at most Q - 1 heap items).  Loads go in groups of 8
+                        // with clamped in-bounds indices and the count applied
+                        // afterwards, so a group waits for memory once
+                        const size_t qb = ((size_t)slot * T + l);
+                        cnt = slots.qcnt[qb];
+                        const unsigned long long* src = slots.qsave + qb * (size_t)Q;
+                        const unsigned long long k0 = src[0];
+                        popped_next = k0 != kKeyNone;
+                        c_next = key_node(k0);
+                        for (int i0 = 0; i0 < Q - 1; i0 += 8) {
+                            unsigned long long x[8];
+    #pragma unroll
+                            for (int j = 0; j < 8; ++j) x[j] = src[1 + min(i0 + j, Q - 2)];
+    #pragma unroll
+                            for (int j = 0; j < 8; ++j)
+                                if (i0 + j < Q - 1) q[(i0 + j) * kWave + L] = i0 + j < cnt ? x[j] : kKeyNone;
+                        }
+                        q[(Q - 1) * kWave + L] = kKeyNone;
                     }
-                    q[(Q - 1) * kWave + L] = kKeyNone;
                 }
             }
         }
-        if (__ballot(has_req) == 0ull) break;
+        // A segment whose ticket gave it nothing to run (a deferred corridor
+        // that still does not fit) passes this round idle and takes the next
+        // ticket, so the workgroup leaves only when no segment has work and
+        // every segment's tickets are spent.  (A loop around the acquisition
+        // would do the same without the idle round, but its code made every
+        // round 9% slower.)
+        if (__ballot(has_req || !no_more) == 0ull) break;
         wave_sync();  // seeds, restored queues and visited writes visible
 
         // ---- pop (first round of a fresh search) + neighbour fetch

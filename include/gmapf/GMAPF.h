@@ -53,6 +53,7 @@ struct Config {
     long long CorridorSlab = 0;  // corridor storage in ints; 0: sized from agents and map
     int SearchesPerWave = 0;     // searches per 64-lane search workgroup; 0: 64 / T
     int SearchTiming = 0;        // time the search launch of every k-th frame; 0: none
+    int SearchWorkgroups = 0;    // batched form: cap on the persistent search grid; 0: automatic
 };
 
 class CuNodesMap {
@@ -99,6 +100,7 @@ class GMAPF {
         cc.corridor_slab = c.CorridorSlab;
         cc.searches_per_wave = c.SearchesPerWave;
         cc.search_timing = c.SearchTiming;
+        cc.search_workgroups = c.SearchWorkgroups;
         check(gmapf_init(h_, &cc));
     }
     void ManualStart() { check(gmapf_manual_start(h_)); }

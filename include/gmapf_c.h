@@ -68,6 +68,11 @@ typedef struct gmapf_config {
                                           frame (frames 0, k, 2k, ...), read by
                                           gmapf_last_frame_ms [2]; 0: no frame.  The two event
                                           nodes lengthen a graph-launched frame by about 20 us */
+    int search_workgroups;             /* batched form: at most this many search workgroups
+                                          (the persistent grid); <= 0: one per
+                                          searches_per_wave searches in flight, up to what
+                                          the GPU holds at once.  A smaller grid makes the
+                                          workgroups take searches by atomic ticket */
 } gmapf_config;
 
 /* Reference CuNodesMap::Node (include/gmapf/misc/CuNodesMap.h:14-21);
@@ -145,8 +150,9 @@ int gmapf_get_stats(gmapf_handle h, long long* out8);
  * without, [2] is the search launch of a frame timed by search_timing (-1 for
  * other frames); host parts [5] enqueue [6] wait for the device [7] readback */
 int gmapf_last_frame_ms(gmapf_handle h, double* out8);
-/* spatial hash of the last frame: keys per agent, and the agents grouped by
- * cell in ascending key order (the order inside a cell is unspecified) */
+/* spatial hash of the last frame that moved agents: cell key per agent (-1 for
+ * a NaN position), and the stable counting sort of the agents by key (ascending
+ * key, ascending agent index inside a cell; *n = agents with a key) */
 int gmapf_get_hash(gmapf_handle h, int* keys, int* order, int* n);
 /* resolved build knobs: [0] T [1] Q [2] P [3] budget [4] search grid [5] mode [6] V */
 int gmapf_get_launch_info(gmapf_handle h, int* out8);

@@ -376,3 +376,17 @@ def test_counting_sort_is_stable(oracle):
     keys, order = w.last_sort()
     valid = np.nonzero(keys >= 0)[0]
     assert np.array_equal(order, valid[np.argsort(keys[valid], kind="stable")])
+
+
+# ------------------------------------------------------- request admission (F1)
+def test_admission_defers_in_ascending_index_and_same_node_goes_idle(oracle):
+    H.check_deferral_in_ascending_index(oracle, gpu=False)
+
+
+@pytest.mark.parametrize("budget", [1, 2, 5])
+def test_admission_fills_freed_slots_in_index_order(oracle, budget):
+    H.check_slots_free_in_index_order(oracle, False, budget)
+
+
+def test_retarget_cancels_search_in_flight(oracle):
+    H.check_retarget_cancels_search_in_flight(oracle, gpu=False)

@@ -48,6 +48,9 @@ def run_pair(oracle, sc, mode=G.MODE_PLAIN, budget=1, frames=None, check_every=1
         w.step(sc.dt)
         if f % check_every == 0 or f == frames - 1:
             H.compare_worlds(g, w, f, check_corridors=(f % (10 * check_every) == 0))
+            H.compare_hash(g, w, f)
+            if f % 60 == 0:
+                print(f"{sc.name} frame {f} equal", flush=True)
     gs, os_ = g.stats(), w.stats()
     for k in ("found", "not_found", "expansions", "discoveries", "corridor_nodes", "rounds", "admitted"):
         assert gs[k] == os_[k], (k, gs[k], os_[k])
@@ -83,31 +86,18 @@ def test_config4_crowd_scaled(oracle):
     assert gs["found"] > 0.9 * sc.n
 
 
-def test_config4_full_deterministic():
-    """65,536 agents, 120 frames, twice: bitwise identical, no NaN."""
+def test_config4_full_equals_oracle(oracle):
+    """The crowd workload at full size: 65,536 agents in two opposing streams,
+    hash cell 4 (5 x 5 stencil), 600 frames.  Avoidance and arrival are
+    discontinuous, so any difference would grow over the run; positions,
+    states, nodes and the hash are compared with the oracle bit for bit every
+    60 frames and at the end."""
     make, keep = locator()
     sc = S.config4(make)
-    outs = []
-    for rep in range(2):
-        cfg = G.Config(agents_max_count=sc.n, agent_radius=sc.radius, agent_speed=sc.speed,
-                       path_finder_parallel_agents=sc.P, path_finder_threads_per_agents=sc.T,
-                       path_finder_queue_capacity=sc.Q, hash_cell=4.0)
-        g = G.GMAPF()
-        g.init(cfg)
-        sc.build(g)
-        for p in sc.starts:
-            g.add_agent(*p)
-        g.set_agent_targets(np.arange(sc.n, dtype=np.int32), sc.targets)
-        print(f"config4 full rep {rep}", flush=True)
-        for f in range(120):
-            g.step(sc.dt)
-        xy, st = g.agents()
-        outs.append((xy.copy(), st.copy(), g.stats()))
-        g.close()
-    assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
-    assert np.array_equal(outs[0][1], outs[1][1])
-    assert not np.isnan(outs[0][0]).any()
-    assert outs[0][2] == outs[1][2]
+    print(f"config4 full {sc.n} agents", flush=True)
+    gs, xy, st = run_pair(oracle, sc, frames=600, check_every=60, hash_cell=4.0)
+    assert gs["found"] > 0.9 * sc.n
+    assert not np.isnan(xy).any()
 
 
 def test_config3_maze_full_corridors_are_tree_paths(oracle):
@@ -117,8 +107,9 @@ def test_config3_maze_full_corridors_are_tree_paths(oracle):
     triangle adjacency graph is a tree and the only simple path between two
     nodes is the corridor PathFinder.h:276-306 describes: every corridor the
     GPU produces must start next to its start node, step between adjacent
-    nodes, never repeat a node and end on the target node.  A sample of the
-    searches is also compared with the oracle (found flag and corridor)."""
+    nodes, never repeat a node and end on the target node.  Every search
+    that ends in frames 0-2 is also compared with the oracle (found flag and
+    corridor)."""
     from gmapf_amd.runner import make_world
     make, keep = locator()
     sc = S.config3(make, n_agents=16384, cells=64)
@@ -151,8 +142,8 @@ def test_config3_maze_full_corridors_are_tree_paths(oracle):
             assert len(np.unique(walk)) == len(walk), (f, i)
             checked += 1
         if f < 3:
-            for i in list(rng.choice(found, size=min(len(found), 120), replace=False)) + \
-                    list(rng.choice(failed, size=min(len(failed), 40), replace=False)):
+            # every search that ended in frames 0-2, found or not
+            for i in list(found) + list(failed):
                 s_node, t_node = int(prev_nodes[i]), int(tgt_nodes[i])
                 ok, oc, _ = om.search(s_node, t_node, T, Q)
                 assert ok == (state[i] == G.MOVE), (f, i)
@@ -168,4 +159,4 @@ def test_config3_maze_full_corridors_are_tree_paths(oracle):
     g.close()
     print(f"maze full: {moved} corridors, {checked} checked as tree paths, {compared} compared with the oracle",
           flush=True)
-    assert moved > 20000 and checked > 10000 and compared > 300
+    assert moved > 20000 and checked > 10000 and compared > 10000

@@ -68,10 +68,12 @@ def test_multi_frame_world_parity(oracle, mode, budget, P, spw):
     g = H.make_gpu(tri, nbr, cfg, starts, targs)
     w = H.make_oracle(oracle, tri, nbr, cfg, starts, targs)
     dt = 1.0 / 60
+    hashed = 0
     for f in range(120):
         g.step(dt)
         w.step(dt)
         H.compare_worlds(g, w, f, check_corridors=(f % 10 == 0))
+        hashed += H.compare_hash(g, w, f)
         if f == 60:  # re-target a few agents mid-run, including searching ones
             for i in range(0, 160, 7):
                 if i % 2 == 0:  # two changes in one frame: the last one wins
@@ -84,6 +86,7 @@ def test_multi_frame_world_parity(oracle, mode, budget, P, spw):
     gs, os_ = g.stats(), w.stats()
     for k in ("found", "not_found", "expansions", "discoveries", "corridor_nodes", "rounds", "admitted"):
         assert gs[k] == os_[k], (k, gs[k], os_[k])
+    assert hashed > 100
 
 
 @pytest.mark.parametrize("mode,budget", [(G.MODE_PLAIN, 0), (G.MODE_BATCHED, 2)])
@@ -176,8 +179,7 @@ def test_search_timing_marks_only_timed_frames_and_changes_no_result(oracle):
 
 def test_hash_groups_agents_by_cell(oracle):
     """The counting sort lists every hashed agent once, grouped by cell in
-    ascending key order (the order inside a cell is unspecified: avoidance
-    takes neighbours by agent index itself)."""
+    ascending key order and by ascending agent index inside a cell."""
     tri, nbr, cfg, starts, targs = _world(44, 30, 30, 400, 4, 16, p=0.1)
     g = H.make_gpu(tri, nbr, cfg, starts, targs)
     for _ in range(5):
@@ -185,7 +187,7 @@ def test_hash_groups_agents_by_cell(oracle):
     keys, order = g.hash_state()
     valid = np.nonzero(keys >= 0)[0]
     assert len(order) == len(valid) and np.array_equal(np.sort(order), valid)
-    assert np.all(np.diff(keys[order]) >= 0)
+    assert np.array_equal(order, valid[np.argsort(keys[valid], kind="stable")])
     assert len(np.unique(keys[valid])) > 10
 
 
@@ -262,8 +264,63 @@ def test_avoidance_dense_cluster_equals_oracle(oracle, spread):
     print(f"dense cluster spread={spread} agents={n}", flush=True)
     g = H.make_gpu(tri, nbr, cfg, starts, targs)
     w = H.make_oracle(oracle, tri, nbr, cfg, starts, targs)
+    hashed = 0
     for f in range(40):
         g.step(1.0 / 60)
         w.step(1.0 / 60)
         H.compare_worlds(g, w, f, check_corridors=(f % 10 == 0))
+        hashed += H.compare_hash(g, w, f)
+    assert hashed > 30
     g.close()
+
+
+# ------------------------------------------------------- request admission (F1)
+def test_admission_defers_in_ascending_index_and_same_node_goes_idle(oracle):
+    """The closed forms the oracle is pinned by, through the C ABI (plain
+    form, P < N: the ranked admission path)."""
+    print("admission closed form: deferral", flush=True)
+    H.check_deferral_in_ascending_index(oracle, gpu=True)
+
+
+@pytest.mark.parametrize("budget", [1, 2, 5])
+def test_admission_fills_freed_slots_in_index_order(oracle, budget):
+    print(f"admission closed form: freed slots budget={budget}", flush=True)
+    H.check_slots_free_in_index_order(oracle, True, budget)
+
+
+def test_retarget_cancels_search_in_flight(oracle):
+    print("admission closed form: re-target cancels", flush=True)
+    H.check_retarget_cancels_search_in_flight(oracle, gpu=True)
+
+
+# ------------------------------------------------------- boundary
+def _hip_current_device():
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    d = ctypes.c_int(-1)
+    assert hip.hipGetDevice(ctypes.byref(d)) == 0
+    return d.value
+
+
+def test_entry_points_leave_the_callers_device_current():
+    """Every entry point that uses the device restores the caller's current
+    device on return (the reference pushes and pops its context around each
+    step, GMAPF.cu:82,127,266,308).  A one-GPU box can only show that the
+    device is unchanged across start, steps, readbacks and destroy."""
+    tri, nbr, cfg, starts, targs = _world(39, 20, 20, 32, 4, 16)
+    before = _hip_current_device()
+    g = H.make_gpu(tri, nbr, cfg, starts, targs)
+    g.manual_start()
+    assert _hip_current_device() == before
+    for f in range(6):
+        g.async_step(1.0 / 60)
+        assert _hip_current_device() == before
+        g.wait_step_end()
+        assert _hip_current_device() == before
+        g.agents(with_nodes=True)
+        g.corridor(0)
+        g.hash_state()
+        assert _hip_current_device() == before
+    g.locate(np.float32([starts[0]]), device=True)
+    g.close()
+    assert _hip_current_device() == before

@@ -51,6 +51,63 @@ def _drive(srv, smaps, engines, frames, lam, seed, corridors):
     return np.concatenate(done_all)
 
 
+def _check_alone(oracle, engines, done, pick, log):
+    """Each picked request alone -- one at a time on a fresh engine of the same
+    map, agent placed on the centroid of the request's start node -- gives the
+    found flag and corridor it got in the mix, and both equal the oracle's
+    search.  Returns the number checked."""
+    by_map = {}
+    for i in pick:
+        by_map.setdefault(int(done[i]["map"]), []).append(done[i])
+    # ticket -> target position, in submission order per (map, agent)
+    tickets_by_key = {}
+    for d in sorted(done, key=lambda d: d["ticket"]):
+        tickets_by_key.setdefault((int(d["map"]), int(d["agent"])), []).append(int(d["ticket"]))
+    target_of = {}
+    for key, ts in tickets_by_key.items():
+        for t, p in zip(ts, log["targets"][key]):
+            target_of[t] = p
+    checked = 0
+    for m, reqs in by_map.items():
+        tri, nbr, cen = engines[m].map_arrays()
+        om = oracle.OracleMap(tri, nbr)
+        starts = np.array([cen[int(d["start_node"])] for d in reqs], dtype=np.float32)
+        alone = G.GMAPF()
+        alone.init(G.Config(agents_max_count=len(reqs), agent_radius=2.0, agent_speed=1.0,
+                            path_finder_parallel_agents=1, path_finder_threads_per_agents=T,
+                            path_finder_queue_capacity=Q, step_mode=G.MODE_PLAIN))
+        alone.fill_map_manually(tri, nbr)
+        for x, y in starts:
+            alone.add_agent(x, y)
+        _, _, nodes = alone.agents(with_nodes=True)
+        assert list(nodes) == [int(d["start_node"]) for d in reqs]
+        alone.manual_start()
+        print(f"alone map {m}: {len(reqs)} tickets", flush=True)
+        for k, d in enumerate(reqs):
+            tx, ty = target_of[int(d["ticket"])]
+            alone.set_agent_targ_pos(k, tx, ty)
+            for _ in range(10):
+                alone.step(1.0 / 60)
+                if alone.get_agent_state(k) != G.SEARCH:
+                    break
+            assert alone.get_agent_state(k) != G.SEARCH
+            mix = log.get(int(d["ticket"]))
+            got = alone.corridor(k) if alone.get_agent_state(k) == G.MOVE else None
+            assert (got is not None) == bool(d["found"])
+            tg = int(om.locate(np.array([[tx, ty]], np.float32))[0])
+            S_node = int(d["start_node"])
+            if S_node != tg:
+                of, oc, _ = om.search(S_node, tg, T, Q)
+                assert of == bool(d["found"])
+                if of:
+                    np.testing.assert_array_equal(oc, got)
+            if d["found"]:
+                np.testing.assert_array_equal(mix, got)
+            checked += 1
+        alone.close()
+    return checked
+
+
 @pytest.mark.parametrize("policy", [ADMIT_FIFO, ADMIT_FAIR])
 def test_results_alone_equal_results_in_the_mix(oracle, policy):
     locs = Locators()
@@ -74,61 +131,9 @@ def test_results_alone_equal_results_in_the_mix(oracle, policy):
     assert (done["done_frame"] > done["admit_frame"]).any(), "no search spanned frames"
     assert done["found"].sum() > 0.5 * len(done)
 
-    # alone: one request at a time on a fresh engine of the same map, agent
-    # placed on the centroid of the request's start node
     rng = np.random.Generator(np.random.PCG64(9))
     pick = rng.choice(len(done), size=min(40, len(done)), replace=False)
-    by_map = {}
-    for i in pick:
-        by_map.setdefault(int(done[i]["map"]), []).append(done[i])
-    # ticket -> target position, in submission order per (map, agent)
-    tickets_by_key = {}
-    for d in sorted(done, key=lambda d: d["ticket"]):
-        tickets_by_key.setdefault((int(d["map"]), int(d["agent"])), []).append(int(d["ticket"]))
-    target_of = {}
-    for key, ts in tickets_by_key.items():
-        for t, p in zip(ts, log["targets"][key]):
-            target_of[t] = p
-    checked = 0
-    for m, reqs in by_map.items():
-        sm = smaps[m]
-        tri, nbr, cen = engines[m].map_arrays()
-        om = oracle.OracleMap(tri, nbr)
-        starts = np.array([cen[int(d["start_node"])] for d in reqs], dtype=np.float32)
-        alone = G.GMAPF()
-        alone.init(G.Config(agents_max_count=len(reqs), agent_radius=2.0, agent_speed=1.0,
-                            path_finder_parallel_agents=1, path_finder_threads_per_agents=T,
-                            path_finder_queue_capacity=Q, step_mode=G.MODE_PLAIN))
-        alone.fill_map_manually(tri, nbr)
-        for x, y in starts:
-            alone.add_agent(x, y)
-        _, _, nodes = alone.agents(with_nodes=True)
-        assert list(nodes) == [int(d["start_node"]) for d in reqs]
-        alone.manual_start()
-        for k, d in enumerate(reqs):
-            tx, ty = target_of[int(d["ticket"])]
-            print(f"alone map {m} ticket {int(d['ticket'])}", flush=True)
-            alone.set_agent_targ_pos(k, tx, ty)
-            for _ in range(10):
-                alone.step(1.0 / 60)
-                if alone.get_agent_state(k) != G.SEARCH:
-                    break
-            assert alone.get_agent_state(k) != G.SEARCH
-            mix = log.get(int(d["ticket"]))
-            got = alone.corridor(k) if alone.get_agent_state(k) == G.MOVE else None
-            assert (got is not None) == bool(d["found"])
-            tg = int(om.locate(np.array([[tx, ty]], np.float32))[0])
-            S_node = int(d["start_node"])
-            if S_node != tg:
-                of, oc, _ = om.search(S_node, tg, T, Q)
-                assert of == bool(d["found"])
-                if of:
-                    np.testing.assert_array_equal(oc, got)
-            if d["found"]:
-                np.testing.assert_array_equal(mix, got)
-            # park the agent: Idle agents keep no request
-            checked += 1
-        alone.close()
+    checked = _check_alone(oracle, engines, done, pick, log)
     assert checked == len(pick)
     srv.close()
     for e in engines:
@@ -212,3 +217,38 @@ def test_fair_policy_bounds_the_wait_of_a_map_behind_a_burst():
     assert wait(la).max() < wait(lf).max()
     print(f"map-1 admission wait, frames: fifo max {wait(lf).max()} mean {wait(lf).mean():.1f}; "
           f"fair max {wait(la).max()} mean {wait(la).mean():.1f}", flush=True)
+
+
+def test_serving_workload_full_size_alone_equals_mix_equals_oracle(oracle):
+    """The serving workload at its size: rect maps 256^2, 1024^2 and 4096^2
+    (1,024 / 16,384 / 32,768 agents, 1,024 / 4,096 / 4,096 slots) resident
+    together, Poisson arrivals at 50,000 requests/s (about 833 a frame) for
+    60 frames under a 2 GiB search-memory budget, then drained.  Counters
+    stay consistent every frame, and 100 sampled tickets per map give the
+    same result alone as in the mix, equal to the oracle's search."""
+    locs = Locators()
+    smaps = serving_maps(locs)
+    assert [sm.size for sm in smaps] == [256.0, 1024.0, 4096.0]
+    engines = [build_map(sm, T, Q, round_budget=128) for sm in smaps]
+    srv = Server(engines, queue_memory_budget=2 << 30, record_start_nodes=True)
+    log = {}
+    lam = 50000.0 / 60.0
+    print("serving full size: arrivals", flush=True)
+    done = _drive(srv, smaps, engines, 60, lam, 17, log)
+    print("serving full size: drain", flush=True)
+    done = np.concatenate([done, _drive(srv, smaps, engines, 2000, 0, 18, log)])
+    st = srv.stats()
+    assert st["pending"] == 0 and st["in_flight"] == 0, st
+    assert st["completed"] == st["submitted"] == len(done)
+    assert st["submitted"] > 40000
+    assert st["peak_live_bytes"] <= 2 << 30
+    rng = np.random.Generator(np.random.PCG64(21))
+    pick = []
+    for m in range(3):
+        idx = np.nonzero(done["map"] == m)[0]
+        assert len(idx) > 1000
+        pick += list(rng.choice(idx, size=100, replace=False))
+    assert _check_alone(oracle, engines, done, pick, log) == 300
+    srv.close()
+    for e in engines:
+        e.close()

@@ -12,10 +12,11 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 
 
-def _run(tri, nbr, starts, targets, mode, slab, frames=800):
+def _run(tri, nbr, starts, targets, mode, slab, frames=800, P=0, wg=0):
     cfg = G.Config(agents_max_count=len(starts), agent_radius=0.5, agent_speed=240.0,
-                   path_finder_parallel_agents=len(starts), path_finder_threads_per_agents=4,
-                   path_finder_queue_capacity=16, step_mode=mode, round_budget=4, corridor_slab=slab)
+                   path_finder_parallel_agents=P or len(starts), path_finder_threads_per_agents=4,
+                   path_finder_queue_capacity=16, step_mode=mode, round_budget=4, corridor_slab=slab,
+                   search_workgroups=wg)
     g = H.make_gpu(tri, nbr, cfg, starts, targets)
     first = {}  # agent -> corridor when it first moved
     seen_move_frame = {}
@@ -29,25 +30,34 @@ def _run(tri, nbr, starts, targets, mode, slab, frames=800):
                 seen_move_frame[i] = f
         if not (st == G.SEARCH).any():
             break
+    # after compactions an agent that stopped moving has no corridor; one that
+    # still moves keeps the one it was given (never another buffer's words)
+    for i in range(len(starts)):
+        c = g.corridor(i)
+        assert c is None or (i in first and np.array_equal(c, first[i])), i
     info = g.slab_info()
     stats = g.stats()
     g.close()
     return first, seen_move_frame, info, stats
 
 
-@pytest.mark.parametrize("mode", [G.MODE_PLAIN, G.MODE_BATCHED])
-def test_small_slab_defers_and_compacts_without_changing_corridors(oracle, mode):
+@pytest.mark.parametrize("mode,P,wg", [(G.MODE_PLAIN, 0, 0), (G.MODE_BATCHED, 0, 0), (G.MODE_BATCHED, 60, 2)])
+def test_small_slab_defers_and_compacts_without_changing_corridors(oracle, mode, P, wg):
+    """P, wg: fewer slots than agents and a persistent grid of wg workgroups
+    (32 segments for 60 slots), so the workgroups take searches by atomic
+    ticket and a segment whose deferred corridor still does not fit takes
+    the next ticket."""
     tri, nbr = H.grid_map(11, 24, 24, cell=4.0, p=0.15)
     pairs = H.connected_pairs(12, nbr, 160)
     cen = tri.reshape(-1, 3, 2).mean(axis=1).astype(np.float32)
     starts = np.array([cen[a] for a, _ in pairs], np.float32)
     targets = np.array([cen[b] for _, b in pairs], np.float32)
     V = len(tri)
-    print(f"slab test mode={mode} V={V} agents={len(pairs)}", flush=True)
-    ref, ref_f, ref_info, ref_stats = _run(tri, nbr, starts, targets, mode, 0)
+    print(f"slab test mode={mode} P={P} wg={wg} V={V} agents={len(pairs)}", flush=True)
+    ref, ref_f, ref_info, ref_stats = _run(tri, nbr, starts, targets, mode, 0, P=P, wg=wg)
     assert ref_info["capacity"] >= len(pairs) * V or ref_info["capacity"] >= len(pairs) * 1024
     assert ref_info["deferred"] == 0 and ref_info["compactions"] == 0
-    small, small_f, info, stats = _run(tri, nbr, starts, targets, mode, 2 * V)
+    small, small_f, info, stats = _run(tri, nbr, starts, targets, mode, 2 * V, P=P, wg=wg)
     assert info["capacity"] == 2 * V
     assert info["deferred"] > 0, info
     assert info["compactions"] > 0, info
