@@ -141,6 +141,7 @@ _SIGS = {
     "gmapf_get_map": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "gmapf_locate": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
     "gmapf_locate_device": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
+    "gmapf_locate_device_noalloc": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
     "gmapf_get_stats": [C.c_void_p, C.c_void_p],
     "gmapf_last_frame_ms": [C.c_void_p, C.c_void_p],
     "gmapf_get_hash": [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)],
@@ -378,10 +379,16 @@ class GMAPF:
         self._check(self._lib.gmapf_get_map(self._h, _ptr(tri), _ptr(nbr), _ptr(cen)))
         return tri, nbr, cen
 
-    def locate(self, xy, device=False):
+    def locate(self, xy, device=False, noalloc=False):
+        """Min containing triangle per point, or -1: on the host, or on the
+        GPU (device=True); noalloc=True stages through the library's fixed
+        per-GPU buffer instead of allocating (at most 2^20 points)."""
         xy = np.ascontiguousarray(np.asarray(xy, dtype=np.float32).reshape(-1, 2))
         out = np.zeros(len(xy), dtype=np.int32)
-        fn = self._lib.gmapf_locate_device if device else self._lib.gmapf_locate
+        if noalloc:
+            fn = self._lib.gmapf_locate_device_noalloc
+        else:
+            fn = self._lib.gmapf_locate_device if device else self._lib.gmapf_locate
         self._check(fn(self._h, len(xy), _ptr(xy), _ptr(out)))
         return out
 

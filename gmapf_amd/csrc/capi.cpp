@@ -101,6 +101,9 @@ struct gmapf_s {
     CrowdParams cp{};
     int search_grid = 0;
     bool ident = false;  // slot i = agent i (P >= N)
+    // this device's staging for gmapf_locate_device_noalloc
+    float2* loc_pts = nullptr;
+    int* loc_out = nullptr;
     int persistent = 0;
     std::vector<void*> dallocs;
     // pinned staging
@@ -255,6 +258,7 @@ struct gmapf_s {
         const int P = cfg.path_finder_parallel_agents;
         const int T = cfg.path_finder_threads_per_agents;
         const int Q = cfg.path_finder_queue_capacity;
+        HIPCHK(locate_scratch(dev, &loc_pts, &loc_out));
         HIPCHK(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
         HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
@@ -1167,6 +1171,25 @@ int gmapf_locate_device(gmapf_handle h, int n, const float* xy, int* out) {
         hipFree(dp);
         if (dout) hipFree(dout);
         HIPCHK(e);
+    });
+}
+
+int gmapf_locate_device_noalloc(gmapf_handle h, int n, const float* xy, int* out) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!h->started) throw ApiError(GMAPF_E_STATE, "not started");
+        if (h->step_going) throw ApiError(GMAPF_E_STATE, "step is running");
+        if (n < 0 || n > kLocateScratchMax)
+            throw ApiError(GMAPF_E_ARG, "gmapf_locate_device_noalloc takes at most " +
+                                            std::to_string(kLocateScratchMax) + " points per call, got " +
+                                            std::to_string(n));
+        if (n == 0) return;
+        if (!xy || !out) throw ApiError(GMAPF_E_ARG, "null points or output");
+        h->use_device();
+        HIPCHK(hipMemcpyAsync(h->loc_pts, xy, (size_t)n * sizeof(float2), hipMemcpyHostToDevice, h->s1));
+        HIPCHK(launch_locate(h->dmap, h->loc_pts, n, h->loc_out, h->s1));
+        HIPCHK(hipMemcpyAsync(out, h->loc_out, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, h->s1));
+        HIPCHK(hipStreamSynchronize(h->s1));
     });
 }
 

@@ -6,6 +6,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <mutex>
 
 #include "device.h"
 #include "kernels.h"
@@ -738,6 +739,14 @@ __global__ __launch_bounds__(kAvoidThreads) void k_avoid_progress(DevMap map, De
     }
 }
 
+// Staging for bulk point location without allocation: the points, then the
+// results.  One copy per GPU, held until the process ends; every call on a
+// GPU shares it.  Host code reaches it only through the address that
+// hipGetSymbolAddress gives for the current device (the array's own name is a
+// host-side stand-in there).
+constexpr size_t kLocateScratchBytes = (size_t)kLocateScratchMax * (sizeof(float2) + sizeof(int));
+static __device__ __align__(256) unsigned char scratch[kLocateScratchBytes];
+
 __global__ void k_locate(DevMap map, const float2* pts, int n, int* out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = locate(map, pts[i].x, pts[i].y);
@@ -806,6 +815,29 @@ hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& 
     hipLaunchKernelGGL(k_avoid_progress, dim3(nblk(ag.n, kAvoidThreads)), dim3(kAvoidThreads), 0, st, map, ag, fr,
                        cp);
     return hipGetLastError();
+}
+
+hipError_t locate_scratch(int device, float2** pts, int** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = p;
+    }
+    *pts = static_cast<float2*>(addr[device]);
+    *out = reinterpret_cast<int*>(static_cast<unsigned char*>(addr[device]) + (size_t)kLocateScratchMax * sizeof(float2));
+    return hipSuccess;
 }
 
 hipError_t launch_locate(const DevMap& map, const float2* pts, int n, int* out, hipStream_t st) {

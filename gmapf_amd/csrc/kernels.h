@@ -37,5 +37,11 @@ hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& 
 
 // bulk point location (min containing triangle or -1)
 hipError_t launch_locate(const DevMap& map, const float2* pts, int n, int* out, hipStream_t st);
+// Points a call of the allocation-free bulk location can stage at once.
+constexpr int kLocateScratchMax = 1 << 20;
+// Device addresses of this library's location staging on `device` (points,
+// then results; kLocateScratchMax of each): looked up once per device, then
+// cached.  Calls that overlap on one device share the staging and race.
+hipError_t locate_scratch(int device, float2** pts, int** out);
 
 }  // namespace gmapf_amd

@@ -142,6 +142,13 @@ int gmapf_map_size(gmapf_handle h, int* nodes);
 int gmapf_get_map(gmapf_handle h, float* tri6, int* nbr3, float* cen2);
 int gmapf_locate(gmapf_handle h, int n, const float* xy, int* out);    /* host, exact */
 int gmapf_locate_device(gmapf_handle h, int n, const float* xy, int* out); /* same, on GPU */
+/* Same result as gmapf_locate_device without allocating: no reference
+ * counterpart, for callers that cannot allocate device memory (code under
+ * stream capture, serving loops that forbid allocation).  Points and results
+ * are staged in a fixed per-GPU buffer of this library (12 MiB, held until
+ * the process ends), so n is at most 1,048,576 (GMAPF_E_ARG beyond).  Calls
+ * that overlap on one GPU, from any handle, share that buffer and race. */
+int gmapf_locate_device_noalloc(gmapf_handle h, int n, const float* xy, int* out);
 /* cumulative device counters: [0] found [1] not found [2] expansions
  * [3] discoveries [4] corridor nodes [5] rounds [6] admitted [7] frames */
 int gmapf_get_stats(gmapf_handle h, long long* out8);
