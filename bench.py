@@ -12,9 +12,15 @@ autotuned pair from gmapf_amd/tuned.json when present.
 Metric: path solves/s (found searches / summed frame wall time, frame =
 AsyncStep entry -> WaitStepEnd return) of the batched step; p50/p99 frame ms,
 the plain step's numbers and their ratio are reported beside it, with the
-search kernel's HBM roofline and a single-core CPU oracle baseline.
+search kernel's HBM roofline and a single-core CPU oracle baseline.  Every
+run first settles for --settle untimed frames (the mix of short and long
+searches in flight shifts for about a hundred frames), then --warmup, then
+the --steps timed frames.  A second pair of runs holds every found agent on
+its corridor for --hold frames before it re-paths, so the timed frames also
+move, hash and steer agents (the full frame of the reference's step).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py --workload {serving, crowd, bowtie, rects}
 """
 from __future__ import annotations
 
@@ -41,6 +47,21 @@ SEARCH_TIMING_EVERY = 8
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """Model, clock and thread count of the host the CPU baseline runs on."""
+    model, mhz = None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model is None:
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("cpu MHz") and mhz is None:
+                    mhz = round(float(line.split(":", 1)[1]), 1)
+    except (OSError, ValueError):
+        pass
+    return dict(cpu_model=model, cpu_mhz=mhz, host_threads=os.cpu_count())
 
 
 def tuned_entry(name, n):
@@ -117,7 +138,7 @@ def cpu_baseline(sc, loc, T, Q, seconds=12.0):
     el = time.process_time() - t0
     return dict(value=round(found / el, 2), unit="solves/s", cores=1, kind="port",
                 sample=f"{n} searches ({rep} passes over the {sc.n} agents' start nodes with log-uniform targets) "
-                       f"of the maze workload, single-threaded oracle, {el:.1f} s CPU")
+                       f"of the maze workload, single-threaded oracle, {el:.1f} s CPU", **host_cpu())
 
 
 def serving_main(args):
@@ -200,28 +221,36 @@ def default_hash_cell(radius):
     return float(np.float32(np.float32(radius) * np.float32(2.5)) * np.float32(1.0 + 1.0 / 64.0))
 
 
+def crowd_frames(sc, cell, warmup, steps, budget):
+    """Frame times (ms) of the crowd workload with the given hash cell."""
+    g = make_world(sc, G.MODE_BATCHED, sc.T, sc.Q, budget, hash_cell=cell)
+    fm = []
+    for f in range(warmup + steps):
+        g.async_step(sc.dt)
+        g.wait_step_end()
+        if f >= warmup:
+            fm.append(g.last_frame_ms()[0])
+        if f % 100 == 0:
+            log(f"crowd cell {cell:.4g} frame {f}: {g.last_frame_ms()[0]:.3f} ms")
+    st_final = g.stats()
+    g.close()
+    return np.array(fm), st_final
+
+
 def crowd_main(args):
     """Crowd workload: 2048^2 with 50 convex obstacles, 65,536 agents in two
-    opposing streams, the library's default hash cell (3 x 3 stencil); one
-    JSON line (agent-steps/s)."""
+    opposing streams, hash cell 4 (5 x 5 stencil) as the workload specifies;
+    the library's default cell (just above cd, 3 x 3 stencil) is measured
+    beside it.  One JSON line (agent-steps/s)."""
     loc = Locators()
     sc = S.config4(loc)
     steps = args.steps if args.steps_given else 600
     T, Q, B = sc.T, sc.Q, args.budget or 256
-    cell = args.hash_cell or default_hash_cell(sc.radius)
+    cell = args.hash_cell or 4.0
     stencil = int(np.floor(np.float32(2.5 * sc.radius) / np.float32(cell))) + 1
-    g = make_world(sc, G.MODE_BATCHED, T, Q, B, hash_cell=cell)
-    fm = []
-    for f in range(args.warmup + steps):
-        g.async_step(sc.dt)
-        g.wait_step_end()
-        if f >= args.warmup:
-            fm.append(g.last_frame_ms()[0])
-        if f % 100 == 0:
-            log(f"crowd frame {f}: {g.last_frame_ms()[0]:.3f} ms")
-    st_final = g.stats()
-    g.close()
-    fm = np.array(fm)
+    fm, st_final = crowd_frames(sc, cell, args.warmup, steps, B)
+    dcell = default_hash_cell(sc.radius)
+    fd, _ = crowd_frames(sc, dcell, args.warmup, steps, B)
     # roofline of the crowd phase: profiled frames (per-phase HIP events)
     gp = make_world(sc, G.MODE_BATCHED, T, Q, B, profile=True, hash_cell=cell)
     tot_b = tot_ms = 0.0
@@ -261,6 +290,10 @@ def crowd_main(args):
             "crowd_phase_ms": round(tot_ms / 30, 4), "crowd_bytes_per_frame": round(tot_b / 30, 1),
             "neighbour_candidates_per_frame": int(np.mean(cands)),
             "searches_found": st_final["found"], "T": T, "Q": Q, "round_budget": B,
+            "default_cell": {"hash_cell": round(dcell, 4), "stencil": "3x3",
+                             "agent_steps_per_s": round(sc.n * len(fd) / (fd.sum() / 1e3), 1),
+                             "p50_ms": round(float(np.percentile(fd, 50)), 4),
+                             "p99_ms": round(float(np.percentile(fd, 99)), 4)},
             "parallelism": "single GPU",
         },
         "roofline": roof,
@@ -294,7 +327,123 @@ def crowd_cpu_baseline(sc, loc, cell, seconds=12.0):
     el = time.process_time() - t0
     return dict(value=round(sc.n * n / el, 1), unit="agent-steps/s", cores=1, kind="port",
                 sample=f"{n} frames of the {sc.n}-agent crowd after the search frame, single-threaded oracle "
-                       f"with the exact hash accelerator, {el:.1f} s CPU")
+                       f"with the exact hash accelerator, {el:.1f} s CPU", **host_cpu())
+
+
+def _oracle_world(sc, g, cell):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    tri, nbr, _ = g.map_arrays()
+    om = pyoracle.OracleMap(tri, nbr)
+    w = pyoracle.OracleWorld(om, agents_max=sc.n, radius=sc.radius, speed=sc.speed, parallel=sc.P,
+                             threads=sc.T, queue_cap=sc.Q, heur_k=sc.K, round_budget=0, hash_cell=cell,
+                             use_grid_locate=1)
+    for x, y in sc.starts:
+        w.add_agent(x, y)
+    for i, (x, y) in enumerate(sc.targets):
+        w.set_target(i, x, y)
+    return w
+
+
+def _side_line(metric, value, unit, steps, warmup, ms, config, roofline, cpu, hib=True):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms, "higher_is_better": hib, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32+int32", "data": "synthetic (seeded SplitMix64 maps and agents)", "config": config,
+            "roofline": roofline, "cpu_baseline": cpu}
+
+
+def bowtie_main(args):
+    """Bow-tie map (100^2, one self-intersecting wall polygon), 64 agents,
+    T=4 Q=16 K=1, 100 frames at dt 1/60: the plain GPU step and the
+    single-core CPU oracle frame by frame, with the frames whose positions,
+    states and nodes are bitwise equal counted.  The GPU frame here is a few
+    launches' latency (64 searches), so the oracle is the faster one."""
+    loc = Locators()
+    sc = S.config1(loc)
+    frames = args.steps if args.steps_given else sc.frames
+    g = make_world(sc, G.MODE_PLAIN, sc.T, sc.Q, 0)
+    w = _oracle_world(sc, g, default_hash_cell(sc.radius))
+    gpu_ms, cpu_ms, equal = [], [], 0
+    for f in range(frames):
+        g.async_step(sc.dt)
+        g.wait_step_end()
+        gpu_ms.append(g.last_frame_ms()[0])
+        t0 = time.perf_counter()
+        w.step(sc.dt)
+        cpu_ms.append((time.perf_counter() - t0) * 1e3)
+        gxy, gst, gnd = g.agents(with_nodes=True)
+        oxy, ost, ond = w.agents()
+        equal += int(np.array_equal(gst, ost) and np.array_equal(gnd, ond)
+                     and np.array_equal(gxy.view(np.uint32), oxy.view(np.uint32)))
+    st = g.stats()
+    g.close()
+    gm, cm = np.array(gpu_ms), np.array(cpu_ms)
+    log(f"bowtie: gpu {gm.mean():.4f} ms/frame, oracle {cm.mean():.4f} ms/frame, {equal}/{frames} frames equal")
+    cfg = {"workload": f"{sc.name}: 100x100 map, bow-tie wall (30,40),(30,60),(60,40),(60,60), {sc.n} agents r=2 v=1, "
+                       f"T={sc.T} Q={sc.Q} K=1, plain step, dt 1/60",
+           "frames": frames, "bitwise_equal_frames": equal, "found": st["found"], "not_found": st["not_found"],
+           "gpu_p50_ms": round(float(np.percentile(gm, 50)), 4), "gpu_p99_ms": round(float(np.percentile(gm, 99)), 4),
+           "cpu_oracle_p50_ms": round(float(np.percentile(cm, 50)), 4),
+           "cpu_oracle_p99_ms": round(float(np.percentile(cm, 99)), 4),
+           "roofline_note": "64 searches on a 100-node map: the frame is launch latency, not memory traffic",
+           "parallelism": "single GPU"}
+    cpu = dict(value=round(float(cm.mean()), 4), unit="ms/frame", cores=1, kind="port",
+               sample=f"the same {frames} frames, single-threaded oracle", **host_cpu())
+    print(json.dumps(_side_line("bow-tie map, 64 agents: plain GPU step frame time beside the single-core CPU oracle",
+                                round(float(gm.mean()), 4), "ms/frame", frames, 0, round(float(gm.mean()), 4), cfg,
+                                None, cpu, hib=False)))
+
+
+def rects_main(args):
+    """Rect map (1000^2, 400 non-overlapping rects with sides 5-40), 2048
+    agents all requesting in frame 0, plain step at T=8 Q=32, 300 frames:
+    solves/s over the run, frame p50/p99, and the search launch of frame 0
+    against its HBM roofline; the single-core oracle on the same run."""
+    loc = Locators()
+    sc = S.config2(loc)
+    frames = args.steps if args.steps_given else sc.frames
+    g = make_world(sc, G.MODE_PLAIN, sc.T, sc.Q, 0, timing=frames + 1)  # times frame 0's search only
+    fm = []
+    st0 = search_ms = None
+    for f in range(frames):
+        g.async_step(sc.dt)
+        g.wait_step_end()
+        ms = g.last_frame_ms()
+        fm.append(ms[0])
+        if f == 0:
+            st0, search_ms = g.stats(), ms[2]
+    st = g.stats()
+    V = g.launch_info()["V"]
+    g.close()
+    fm = np.array(fm)
+    b0 = 48.0 * st0["expansions"] + 16.0 * st0["discoveries"] + 8.0 * st0["corridor_nodes"]
+    gbs = b0 / (search_ms / 1e3) / 1e9
+    roof = dict(bound="hbm", achieved=round(gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 5),
+                traffic=None)
+    cpu = None
+    if not args.skip_cpu:
+        w = _oracle_world(sc, loc.keep[-1], default_hash_cell(sc.radius))
+        t0 = time.process_time()
+        n = 0
+        while n < frames and time.process_time() - t0 < 15.0:
+            w.step(sc.dt)
+            n += 1
+        el = time.process_time() - t0
+        found = w.stats()["found"]
+        cpu = dict(value=round(found / el, 2), unit="solves/s", cores=1, kind="port",
+                   sample=f"{n} of the {frames} frames ({found} searches found in frame 0, then the crowd), "
+                          f"single-threaded oracle, {el:.3f} s CPU", **host_cpu())
+    solves = st["found"] / (fm.sum() / 1e3)
+    cfg = {"workload": f"{sc.name}: 1000x1000 map, 400 seeded non-overlapping rects (sides 5-40), {sc.n} agents r=2 v=1 "
+                       f"all requesting in frame 0, T={sc.T} Q={sc.Q}, plain step, dt 1/60",
+           "navmesh_nodes": V, "frames": frames, "found": st["found"], "not_found": st["not_found"],
+           "p50_ms": round(float(np.percentile(fm, 50)), 4), "p99_ms": round(float(np.percentile(fm, 99)), 4),
+           "frame0_ms": round(float(fm[0]), 4), "frame0_solves_per_s": round(st0["found"] / (fm[0] / 1e3), 1),
+           "frame0_search_kernel_ms": round(float(search_ms), 4), "parallelism": "single GPU"}
+    log("rects", cfg)
+    print(json.dumps(_side_line("rect map, 2048 agents: plain-step path solves/s over 300 frames, with p50/p99 frame ms",
+                                round(solves, 1), "solves/s", frames, 0, round(float(fm.mean()), 4), cfg, roof, cpu)))
 
 
 def main():
@@ -302,7 +451,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=("maze", "serving", "crowd"), default="maze")
+    ap.add_argument("--workload", choices=("maze", "serving", "crowd", "bowtie", "rects"), default="maze")
+    ap.add_argument("--settle", type=int, default=100, help="untimed frames before the warm-up ones")
+    ap.add_argument("--hold", type=int, default=30, help="moving variant: frames a found agent moves first")
+    ap.add_argument("--skip-moving", action="store_true")
     ap.add_argument("--agents", type=int, default=16384)
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--T", type=int, default=0)
@@ -324,6 +476,10 @@ def main():
         return serving_main(args)
     if args.workload == "crowd":
         return crowd_main(args)
+    if args.workload == "bowtie":
+        return bowtie_main(args)
+    if args.workload == "rects":
+        return rects_main(args)
     t_all = time.time()
     loc = Locators()
     sc = S.config3(loc, n_agents=args.agents, cells=args.cells)
@@ -332,21 +488,43 @@ def main():
     T, Q, B = args.T or T, args.Q or Q, args.budget or B
     log(f"workload {sc.name}: {sc.n} agents, T={T} Q={Q} budget={B} (autotuned={tp is not None})")
     plain = None
+    settle = max(0, args.settle)
     if not args.skip_plain:
-        plain = run_frames(sc, G.MODE_PLAIN, T, Q, 0, args.warmup, args.steps, log=log)
+        plain = run_frames(sc, G.MODE_PLAIN, T, Q, 0, args.warmup, args.steps, log=log, settle=settle)
         log("plain", {k: v for k, v in plain.items() if k not in ("stats", "launch")})
     # HIP events around the search launch of every k-th frame (their two
     # graph nodes add ~20 us to a frame, so the other frames carry none)
     tk = max(1, min(SEARCH_TIMING_EVERY, args.steps))
-    bat = run_frames(sc, G.MODE_BATCHED, T, Q, B, args.warmup, args.steps, log=log, timing=tk)
+    bat = run_frames(sc, G.MODE_BATCHED, T, Q, B, args.warmup, args.steps, log=log, timing=tk, settle=settle)
     log("batched", {k: v for k, v in bat.items() if k not in ("stats", "launch")})
+    # the complete frame: found agents move `hold` frames before re-pathing,
+    # so the timed frames also move, hash, steer and advance agents
+    moving = None
+    if not args.skip_moving and args.hold > 0:
+        mb = run_frames(sc, G.MODE_BATCHED, T, Q, B, args.warmup, args.steps, settle=settle, hold=args.hold)
+        mp = None if args.skip_plain else run_frames(sc, G.MODE_PLAIN, T, Q, 0, args.warmup, args.steps,
+                                                      settle=settle, hold=args.hold)
+        moving = {"hold_frames": args.hold, "moving_agents_mean": round(mb["moving_agents_mean"], 1),
+                  "batched_solves_per_s": round(mb["solves_per_s"], 1), "batched_mean_ms": round(mb["mean_ms"], 4),
+                  "plain_mean_ms": round(mp["mean_ms"], 4) if mp else None,
+                  "plain_p50_ms": round(mp["p50_ms"], 4) if mp else None,
+                  "batched_p50_ms": round(mb["p50_ms"], 4), "batched_p99_ms": round(mb["p99_ms"], 4),
+                  "batched_expansions_per_s": round(mb["expansions_per_s"], 1),
+                  "plain_solves_per_s": round(mp["solves_per_s"], 1) if mp else None,
+                  "plain_p99_ms": round(mp["p99_ms"], 4) if mp else None,
+                  "batched_over_plain": round(mb["solves_per_s"] / mp["solves_per_s"], 3) if mp else None,
+                  "batched_over_plain_expansions": (round(mb["expansions_per_s"] / mp["expansions_per_s"], 3)
+                                                    if mp else None)}
+        log("moving", moving)
     # the autotuner's fastest point, when it is not the reported one
     fastest = None
     te = tuned_entry(sc.name, sc.n) if tp and not (args.T or args.Q or args.budget) else None
     fe = te.get("fastest") if te else None
     if fe and (fe["T"], fe["Q"], fe["budget"]) != (T, Q, B):
-        fb = run_frames(sc, G.MODE_BATCHED, fe["T"], fe["Q"], fe["budget"], args.warmup, args.steps, timing=tk)
-        fp = None if args.skip_plain else run_frames(sc, G.MODE_PLAIN, fe["T"], fe["Q"], 0, args.warmup, args.steps)
+        fb = run_frames(sc, G.MODE_BATCHED, fe["T"], fe["Q"], fe["budget"], args.warmup, args.steps, timing=tk,
+                        settle=settle)
+        fp = None if args.skip_plain else run_frames(sc, G.MODE_PLAIN, fe["T"], fe["Q"], 0, args.warmup, args.steps,
+                                                     settle=settle)
         fastest = dict(T=fe["T"], Q=fe["Q"], round_budget=fe["budget"], solves_per_s=round(fb["solves_per_s"], 1),
                        p99_ms=round(fb["p99_ms"], 4),
                        plain_solves_per_s=round(fp["solves_per_s"], 1) if fp else None,
@@ -380,6 +558,11 @@ def main():
             "plain_p99_ms": round(plain["p99_ms"], 4) if plain else None,
             "batched_over_plain": (round(bat["solves_per_s"] / plain["solves_per_s"], 3)
                                    if plain and plain["solves_per_s"] else None),
+            "plain_expansions_per_s": round(plain["expansions_per_s"], 1) if plain else None,
+            "batched_over_plain_expansions": (round(bat["expansions_per_s"] / plain["expansions_per_s"], 3)
+                                              if plain and plain["expansions_per_s"] else None),
+            "settle_frames": settle,
+            "moving_variant": moving,
             "search_kernel_ms_per_launch": round(roof["ms_per_launch"], 4),
             "search_kernel_timed_launches": bat["search_frames"],
             "autotune_choice": ("fastest point with batched >= 5x plain at p99 <= 4 ms"
