@@ -72,9 +72,10 @@ def tuned_entry(name, n):
         return None
 
 
-def tuned_triple(name, n):
+def tuned_point(name, n):
+    """(T, Q, budget, searches per workgroup) of the tuned point, or None."""
     e = tuned_entry(name, n)
-    return (int(e["T"]), int(e["Q"]), int(e["budget"])) if e else None
+    return (int(e["T"]), int(e["Q"]), int(e["budget"]), int(e.get("spw", 0))) if e else None
 
 
 def measured_traffic(T, Q, budget, n):
@@ -483,10 +484,12 @@ def main():
     t_all = time.time()
     loc = Locators()
     sc = S.config3(loc, n_agents=args.agents, cells=args.cells)
-    tp = tuned_triple(sc.name, sc.n)
-    T, Q, B = tp if tp else (8, 32, 192)
+    tp = tuned_point(sc.name, sc.n)
+    T, Q, B, spw = tp if tp else (8, 32, 192, 0)
     T, Q, B = args.T or T, args.Q or Q, args.budget or B
-    log(f"workload {sc.name}: {sc.n} agents, T={T} Q={Q} budget={B} (autotuned={tp is not None})")
+    if args.T or args.Q:
+        spw = 0
+    log(f"workload {sc.name}: {sc.n} agents, T={T} Q={Q} budget={B} spw={spw} (autotuned={tp is not None})")
     plain = None
     settle = max(0, args.settle)
     if not args.skip_plain:
@@ -495,13 +498,15 @@ def main():
     # HIP events around the search launch of every k-th frame (their two
     # graph nodes add ~20 us to a frame, so the other frames carry none)
     tk = max(1, min(SEARCH_TIMING_EVERY, args.steps))
-    bat = run_frames(sc, G.MODE_BATCHED, T, Q, B, args.warmup, args.steps, log=log, timing=tk, settle=settle)
+    bat = run_frames(sc, G.MODE_BATCHED, T, Q, B, args.warmup, args.steps, log=log, timing=tk, settle=settle,
+                     spw=spw)
     log("batched", {k: v for k, v in bat.items() if k not in ("stats", "launch")})
     # the complete frame: found agents move `hold` frames before re-pathing,
     # so the timed frames also move, hash, steer and advance agents
     moving = None
     if not args.skip_moving and args.hold > 0:
-        mb = run_frames(sc, G.MODE_BATCHED, T, Q, B, args.warmup, args.steps, settle=settle, hold=args.hold)
+        mb = run_frames(sc, G.MODE_BATCHED, T, Q, B, args.warmup, args.steps, settle=settle, hold=args.hold,
+                        spw=spw)
         mp = None if args.skip_plain else run_frames(sc, G.MODE_PLAIN, T, Q, 0, args.warmup, args.steps,
                                                       settle=settle, hold=args.hold)
         moving = {"hold_frames": args.hold, "moving_agents_mean": round(mb["moving_agents_mean"], 1),
@@ -520,12 +525,13 @@ def main():
     fastest = None
     te = tuned_entry(sc.name, sc.n) if tp and not (args.T or args.Q or args.budget) else None
     fe = te.get("fastest") if te else None
-    if fe and (fe["T"], fe["Q"], fe["budget"]) != (T, Q, B):
+    if fe and (fe["T"], fe["Q"], fe["budget"], fe.get("spw", 0)) != (T, Q, B, spw):
         fb = run_frames(sc, G.MODE_BATCHED, fe["T"], fe["Q"], fe["budget"], args.warmup, args.steps, timing=tk,
-                        settle=settle)
+                        settle=settle, spw=fe.get("spw", 0))
         fp = None if args.skip_plain else run_frames(sc, G.MODE_PLAIN, fe["T"], fe["Q"], 0, args.warmup, args.steps,
                                                      settle=settle)
-        fastest = dict(T=fe["T"], Q=fe["Q"], round_budget=fe["budget"], solves_per_s=round(fb["solves_per_s"], 1),
+        fastest = dict(T=fe["T"], Q=fe["Q"], round_budget=fe["budget"], searches_per_wave=fe.get("spw", 0),
+                       solves_per_s=round(fb["solves_per_s"], 1),
                        p99_ms=round(fb["p99_ms"], 4),
                        plain_solves_per_s=round(fp["solves_per_s"], 1) if fp else None,
                        batched_over_plain=round(fb["solves_per_s"] / fp["solves_per_s"], 3) if fp else None)
@@ -550,7 +556,7 @@ def main():
             "workload": f"{sc.name} spanning-tree maze (64-unit cells, 16-wide corridors), {sc.n} agents, "
                         "log-uniform 10-4000 start-target distance, continuous re-path",
             "navmesh_nodes": bat["launch"]["V"],
-            "T": T, "Q": Q, "round_budget": B, "autotuned": tp is not None,
+            "T": T, "Q": Q, "round_budget": B, "searches_per_wave": spw or 64 // T, "autotuned": tp is not None,
             "batched_p50_ms": round(bat["p50_ms"], 4), "batched_p99_ms": round(bat["p99_ms"], 4),
             "batched_expansions_per_s": round(bat["expansions_per_s"], 1),
             "plain_solves_per_s": round(plain["solves_per_s"], 1) if plain else None,

@@ -1,26 +1,34 @@
-"""Autotuner for the batched step: threads per search T, queue capacity Q and
-the per-frame round budget, chosen by measurement on the target workload.
+"""Autotuner for the batched step: threads per search T, queue capacity Q,
+the per-frame round budget and the block shape (searches per 64-lane
+workgroup), chosen by measurement on the target workload.
 
-For every (T, Q) whose LDS footprint fits, continuous re-path runs of the
-batched form at each round budget of BUDGETS, and one run of the plain form,
-measure solves/s (found searches only: a small Q that fails fast does not
-count) and the p99 frame time over the benchmark's own window (5 warm-up +
-120 frames: the mix of short and long searches in flight shifts towards long
-ones over the first hundred frames, so shorter probes favour small budgets).
-Plain and batched run the same (T, Q), so their corridors are identical.
+Stage 1: for every (T, Q) whose LDS footprint fits, continuous re-path runs
+of the batched form at each round budget of BUDGETS, and one run of the plain
+form, measure solves/s (found searches only: a small Q that fails fast does
+not count) and the p99 frame time over the benchmark's own window (100
+settle + 5 warm-up + 120 frames: the mix of short and long searches in flight
+shifts towards long ones over the first hundred frames).  Plain and batched
+run the same (T, Q), so their corridors are identical.
 
-Two points are kept:
+Stage 2 (block shape): the best few stage-1 points are run again with fewer
+searches per workgroup (64/T halved, quartered), which puts more workgroups
+on each SIMD and leaves lanes idle (the reference fixes 128-thread blocks,
+PathFinder.h:560-565).
+
+Selection (``select``): two points are kept,
   * "target": the highest batched solves/s whose p99 frame is within
     --target-ms and whose solves/s is at least --ratio times the plain
     step's at the same (T, Q) (the batching goal);
   * "fastest": the highest batched solves/s within --target-ms, whatever
-    the ratio.
-The best few candidates of each are re-measured before the choice.  Results
-are cached per (workload, agents) in gmapf_amd/tuned.json; the benchmark
-reports the "target" point (or "fastest" when no point reaches the ratio)
-and measures the other beside it.
+    the ratio (when no point is within --target-ms: the lowest p99).
+The best `top` candidates of each are re-measured and the gates are applied to
+the re-measured figures before the choice.  Results are cached per
+(workload, agents) in gmapf_amd/tuned.json; the benchmark reports the
+"target" point (or "fastest" when no point reaches the ratio) and measures
+the other beside it.
 
     python -m gmapf_amd.autotune [--agents 16384] [--target-ms 4.0] [--ratio 5.0]
+                                 [--T 2,4,8] [--Q 16,32] [--budgets 96,128,192]
 """
 from __future__ import annotations
 
@@ -39,6 +47,7 @@ T_SET = (2, 4, 8, 16, 32, 64)
 Q_SET = (16, 32, 64, 128, 256)
 LDS_BYTES = 160 * 1024
 BUDGETS = (64, 96, 128, 192, 256, 384, 512)
+SETTLE = 100
 
 
 def lds_fits(Q):
@@ -46,33 +55,31 @@ def lds_fits(Q):
     return 64 * (Q + 5) * 8 + 10 * 1024 <= LDS_BYTES
 
 
-def _confirm(sc, rows, warmup, steps, log):
-    for r in rows:
-        if "solves_confirm" in r:
-            continue
-        c = run_frames(sc, MODE_BATCHED, r["T"], r["Q"], r["budget"], warmup, steps)
-        r["solves_confirm"] = c["solves_per_s"]
-        r["p99_confirm"] = c["p99_ms"]
-        log(f"[tune] confirm T={r['T']} Q={r['Q']} B={r['budget']}: {c['solves_per_s']:.4g} solves/s "
-            f"p99 {c['p99_ms']:.2f} ms (plain {r['plain_solves_per_s']:.4g})")
+def shape_options(T):
+    """Searches per workgroup to try beside the full 64 / T (0 in a row)."""
+    full = 64 // T
+    return [s for s in (full // 2, full // 4) if s >= 1]
 
 
-def tune(sc, target_ms=4.0, ratio=5.0, budgets=BUDGETS, warmup=5, steps=120, top=3, log=print):
+def measure_grid(sc, t_set=T_SET, q_set=Q_SET, budgets=BUDGETS, target_ms=4.0, warmup=5, steps=120, log=print):
+    """Stage 1: the plain form once and the batched form at each budget, per
+    (T, Q).  Rows: T, Q, budget, spw (0: 64 / T), solves/s, p50/p99, the
+    plain step's solves/s and p99, and the ratio."""
     table = []
-    for T in T_SET:
-        for Q in Q_SET:
+    for T in t_set:
+        for Q in q_set:
             if not lds_fits(Q):
                 continue
             try:
-                p = run_frames(sc, MODE_PLAIN, T, Q, 0, warmup, steps)
+                p = run_frames(sc, MODE_PLAIN, T, Q, 0, warmup, steps, settle=SETTLE)
             except GmapfError as e:  # does not fit this device
                 log(f"[tune] T={T} Q={Q}: {e}")
                 continue
             log(f"[tune] T={T:2d} Q={Q:3d} plain: {p['solves_per_s']:.4g} solves/s p99 {p['p99_ms']:.2f} ms")
             for B in budgets:
                 t0 = time.time()
-                r = run_frames(sc, MODE_BATCHED, T, Q, B, warmup, steps)
-                row = dict(T=T, Q=Q, budget=B, solves_per_s=r["solves_per_s"], p50_ms=r["p50_ms"],
+                r = run_frames(sc, MODE_BATCHED, T, Q, B, warmup, steps, settle=SETTLE)
+                row = dict(T=T, Q=Q, budget=B, spw=0, solves_per_s=r["solves_per_s"], p50_ms=r["p50_ms"],
                            p99_ms=r["p99_ms"], found=r["found"], plain_solves_per_s=p["solves_per_s"],
                            plain_p99_ms=p["p99_ms"], ratio=r["solves_per_s"] / max(p["solves_per_s"], 1e-9),
                            secs=round(time.time() - t0, 1))
@@ -81,10 +88,37 @@ def tune(sc, target_ms=4.0, ratio=5.0, budgets=BUDGETS, warmup=5, steps=120, top
                     f"p50 {r['p50_ms']:.2f} p99 {r['p99_ms']:.2f} ms ratio {row['ratio']:.2f}")
                 if r["p50_ms"] > target_ms:  # larger budgets only make frames longer
                     break
+    return table
+
+
+def measure_shapes(sc, table, target_ms=4.0, best=3, warmup=5, steps=120, log=print):
+    """Stage 2: the `best` stage-1 points within target_ms again with fewer
+    searches per workgroup."""
+    rows = []
+    cands = sorted([r for r in table if r["spw"] == 0 and r["p99_ms"] <= target_ms],
+                   key=lambda r: -r["solves_per_s"])[:best]
+    for base in cands:
+        for spw in shape_options(base["T"]):
+            r = run_frames(sc, MODE_BATCHED, base["T"], base["Q"], base["budget"], warmup, steps, settle=SETTLE,
+                           spw=spw)
+            row = dict(base, spw=spw, solves_per_s=r["solves_per_s"], p50_ms=r["p50_ms"], p99_ms=r["p99_ms"],
+                       found=r["found"], ratio=r["solves_per_s"] / max(base["plain_solves_per_s"], 1e-9))
+            rows.append(row)
+            log(f"[tune] T={base['T']:2d} Q={base['Q']:3d} B={base['budget']:3d} spw={spw:2d}: "
+                f"{r['solves_per_s']:.4g} solves/s p99 {r['p99_ms']:.2f} ms")
+    return rows
+
+
+def select(table, target_ms, ratio, confirm, top=3):
+    """Choose "target" and "fastest" from measured rows.  confirm(row)
+    re-measures a candidate and returns (solves/s, p99 ms); the gates apply
+    to the re-measured figures.  Returns (target or None, fastest)."""
 
     def pick(cands, need):
         ranked = sorted(cands, key=lambda r: -r["solves_per_s"])[:top]
-        _confirm(sc, ranked, warmup, steps, log)
+        for r in ranked:
+            if "solves_confirm" not in r:
+                r["solves_confirm"], r["p99_confirm"] = confirm(r)
         ok = [r for r in ranked
               if r["p99_confirm"] <= target_ms and r["solves_confirm"] >= need * r["plain_solves_per_s"]]
         return max(ok, key=lambda r: r["solves_confirm"]) if ok else None
@@ -95,13 +129,29 @@ def tune(sc, target_ms=4.0, ratio=5.0, budgets=BUDGETS, warmup=5, steps=120, top
     target = pick(with_ratio, ratio) if with_ratio else None
     if fastest is None:
         fastest = min(table, key=lambda r: r["p99_ms"])
+    return target, fastest
+
+
+def tune(sc, target_ms=4.0, ratio=5.0, t_set=T_SET, q_set=Q_SET, budgets=BUDGETS, warmup=5, steps=120, top=3,
+         log=print):
+    table = measure_grid(sc, t_set, q_set, budgets, target_ms, warmup, steps, log)
+    table += measure_shapes(sc, table, target_ms, top, warmup, steps, log)
+
+    def confirm(r):
+        c = run_frames(sc, MODE_BATCHED, r["T"], r["Q"], r["budget"], warmup, steps, settle=SETTLE, spw=r["spw"])
+        log(f"[tune] confirm T={r['T']} Q={r['Q']} B={r['budget']} spw={r['spw']}: {c['solves_per_s']:.4g} "
+            f"solves/s p99 {c['p99_ms']:.2f} ms (plain {r['plain_solves_per_s']:.4g})")
+        return c["solves_per_s"], c["p99_ms"]
+
+    target, fastest = select(table, target_ms, ratio, confirm, top)
     return target, fastest, table
 
 
 def _entry(r):
     if r is None:
         return None
-    return dict(T=r["T"], Q=r["Q"], budget=r["budget"], solves_per_s=r.get("solves_confirm", r["solves_per_s"]),
+    return dict(T=r["T"], Q=r["Q"], budget=r["budget"], spw=r.get("spw", 0),
+                solves_per_s=r.get("solves_confirm", r["solves_per_s"]),
                 p99_ms=r.get("p99_confirm", r["p99_ms"]), plain_solves_per_s=r["plain_solves_per_s"],
                 ratio=r.get("solves_confirm", r["solves_per_s"]) / max(r["plain_solves_per_s"], 1e-9))
 
@@ -113,21 +163,26 @@ def main():
     ap.add_argument("--target-ms", type=float, default=4.0)
     ap.add_argument("--ratio", type=float, default=5.0)
     ap.add_argument("--out", default=TUNE_FILE)
+    ap.add_argument("--T", default="", help="comma-separated subset of threads per search")
+    ap.add_argument("--Q", default="", help="comma-separated subset of queue capacities")
+    ap.add_argument("--budgets", default="", help="comma-separated round budgets")
     args = ap.parse_args()
+    ints = lambda v, d: tuple(int(x) for x in v.split(",")) if v else d
     loc = Locators()
     sc = S.config3(loc, n_agents=args.agents, cells=args.cells)
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
 
-    target, fastest, table = tune(sc, args.target_ms, args.ratio, log=log)
+    target, fastest, table = tune(sc, args.target_ms, args.ratio, ints(args.T, T_SET), ints(args.Q, Q_SET),
+                                  ints(args.budgets, BUDGETS), log=log)
     try:
         with open(args.out) as f:
             d = json.load(f)
     except (OSError, ValueError):
         d = {}
     chosen = target or fastest
-    d[f"{sc.name}/{sc.n}"] = dict(T=chosen["T"], Q=chosen["Q"], budget=chosen["budget"],
+    d[f"{sc.name}/{sc.n}"] = dict(T=chosen["T"], Q=chosen["Q"], budget=chosen["budget"], spw=chosen.get("spw", 0),
                                  target=_entry(target), fastest=_entry(fastest), target_ms=args.target_ms,
                                  ratio_goal=args.ratio, table=table)
     with open(args.out, "w") as f:
