@@ -578,9 +578,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                         c_next = key_node(k0);
                         for (int i0 = 0; i0 < Q - 1; i0 += 8) {
                             unsigned long long x[8];
-    #pragma unroll
+#pragma unroll
                             for (int j = 0; j < 8; ++j) x[j] = src[1 + min(i0 + j, Q - 2)];
-    #pragma unroll
+#pragma unroll
                             for (int j = 0; j < 8; ++j)
                                 if (i0 + j < Q - 1) q[(i0 + j) * kWave + L] = i0 + j < cnt ? x[j] : kKeyNone;
                         }
