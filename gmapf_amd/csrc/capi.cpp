@@ -454,10 +454,9 @@ struct gmapf_s {
         dfr.hkey = dalloc<int>(Nn, "hash keys");
         dfr.hcount = dalloc<int>((size_t)dfr.ncell + 1, "hash counts");
         HIPCHK(launch_fill(dfr.hcount, 0, dfr.ncell + 1, s1));
-        dfr.hstart = dalloc<int>((size_t)dfr.ncell + 1, "hash starts");
-        HIPCHK(launch_fill(dfr.hstart, 0, dfr.ncell + 1, s1));  // no agent hashed before the first mover
+        dfr.hstart = dalloc<int>((size_t)dfr.ncell + 2, "hash starts");
+        HIPCHK(launch_fill(dfr.hstart, 0, dfr.ncell + 2, s1));  // no agent hashed before the first mover
         dfr.hslot = dalloc<int>(Nn, "hash arrival slots");
-        dfr.hfill = dalloc<int>(Nn, "hash fill");
         dfr.horder = dalloc<int>(Nn, "hash order");
         dfr.hent = dalloc<float4>(Nn, "hash positions");
         dfr.hblk = dalloc<int>(dfr.ncell / scan_block + 2, "hash scan partials");

@@ -127,11 +127,11 @@ struct DevFrame {
     // spatial hash
     int* hkey;        // N: cell key of the moved position, -1 for a NaN position
     int* hslot;       // N: arrival slot of the agent in its cell (histogram atomic)
-    int* hcount;      // ncell + 1, zero between frames
-    int* hstart;      // ncell + 1
-    int* hfill;       // N: agents grouped by cell, arrival order inside a cell
-    int* horder;      // N: agents grouped by cell, ascending index inside a cell
-    float4* hent;     // N: {x, y, bits of the agent index, 0} in horder's order
+    int* hcount;      // ncell + 1 (cell ncell: agents with a NaN position), zero between frames
+    int* hstart;      // ncell + 2: first entry of each cell; [ncell] = agents hashed, [ncell + 1] = N
+    int* horder;      // N: hashed agents grouped by cell, ascending index inside a cell
+    float4* hent;     // N: {x, y, bits of the agent index, 0} grouped by cell,
+                      // arrival order inside a cell, then the NaN agents
     int* hblk;        // scan partials over cells
     int ncell;
     float* stepv;                // device copy of speed * dt, made by the admission kernel

@@ -1,7 +1,9 @@
 // k_frame.hip -- per-frame kernels around the search: request admission,
 // corridor build, move, spatial hash (integer counting sort), avoidance and
-// path progress.  Every kernel is deterministic: orders come from prefix sums
-// over agent / slot indices, never from atomic arrival order.
+// path progress.  Every result is deterministic: orders come from prefix sums
+// and ranks over agent / slot indices; where an atomic's arrival order places
+// data (the hash's position copy inside a cell), the reader does not depend
+// on it.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -126,14 +128,17 @@ __device__ __forceinline__ float get_sign(float p1x, float p1y, float p2x, float
 }
 
 // CuNodesMap.h:260-270
-__device__ __forceinline__ bool in_tri(const DevMap& m, int t, float px, float py) {
-    const float2 a = m.tri[3 * (size_t)t], b = m.tri[3 * (size_t)t + 1], c = m.tri[3 * (size_t)t + 2];
+__device__ __forceinline__ bool in_tri_v(float2 a, float2 b, float2 c, float px, float py) {
     float d1 = get_sign(px, py, a.x, a.y, b.x, b.y);
     float d2 = get_sign(px, py, b.x, b.y, c.x, c.y);
     float d3 = get_sign(px, py, c.x, c.y, a.x, a.y);
     bool neg = (d1 < 0) || (d2 < 0) || (d3 < 0);
     bool pos = (d1 > 0) || (d2 > 0) || (d3 > 0);
     return !(neg && pos);
+}
+
+__device__ __forceinline__ bool in_tri(const DevMap& m, int t, float px, float py) {
+    return in_tri_v(m.tri[3 * (size_t)t], m.tri[3 * (size_t)t + 1], m.tri[3 * (size_t)t + 2], px, py);
 }
 
 __device__ __forceinline__ int cell_of(float v, float o, float inv, int n) {
@@ -534,11 +539,12 @@ __global__ void k_slab_finish(DevAgents ag, DevSlots sl, DevFrame fr) {
 //                  return value (the agent's arrival slot in its cell) is kept
 //   scan           exclusive prefix of the counts -> cell starts; the counts
 //                  are zeroed as they are read, ready for the next frame
-//   k_hash_place   each agent to start + arrival slot (cell groups complete,
-//                  order inside a cell is arrival order)
-//   k_hash_rank    each agent's rank in its cell = members with a smaller
-//                  index; the final order and a copy of the positions in that
-//                  order (the avoidance pass reads only the copy)
+//   k_hash_place   each agent's position to start + arrival slot (cell
+//                  groups complete, arrival order inside a cell); the
+//                  avoidance pass reads only this copy
+//   k_avoid_progress  while it reads its own cell, each agent counts the
+//                  members with a smaller index: its rank, and its place in
+//                  the final order (horder)
 // AgentsMover.h:57-70 (move), then the agent's hash cell from its moved
 // position, counted into hcount (all zero at the start of a frame)
 __global__ void k_move_key(DevMap map, DevAgents ag, DevFrame fr) {
@@ -564,30 +570,24 @@ __global__ void k_move_key(DevMap map, DevAgents ag, DevFrame fr) {
         int cx = cell_of(p.x, map.hx0, map.hinv, map.hnx);
         int cy = cell_of(p.y, map.hy0, map.hinv, map.hny);
         key = cy * map.hnx + cx;
-        fr.hslot[i] = atomicAdd(&fr.hcount[key], 1);
     }
+    // agents with a NaN position go to the extra cell ncell, after every
+    // hashed agent (key -1: not in the hash's order)
+    fr.hslot[i] = atomicAdd(&fr.hcount[key >= 0 ? key : fr.ncell], 1);
     fr.hkey[i] = key;
 }
 
+__device__ __forceinline__ int cell_start(const DevFrame& fr, int c) { return fr.hstart[c]; }
+
+// The agent's entry at its cell's start plus its arrival slot: the cells are
+// grouped in key order, the agents of a cell in arrival order (the avoidance
+// pass takes a cell's agents by index itself).
 __global__ void k_hash_place(DevAgents ag, DevFrame fr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ag.n || fr.counters[5] == 0) return;
     const int key = fr.hkey[i];
-    if (key < 0) return;
-    fr.hfill[fr.hstart[key] + fr.hslot[i]] = i;
-}
-
-__global__ void k_hash_rank(DevAgents ag, DevFrame fr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ag.n || fr.counters[5] == 0) return;
-    const int key = fr.hkey[i];
-    if (key < 0) return;
-    const int b = fr.hstart[key], e = fr.hstart[key + 1];
-    int r = 0;
-    for (int q = b; q < e; ++q) r += fr.hfill[q] < i ? 1 : 0;
     const float2 p = ag.pos[i];
-    fr.horder[b + r] = i;
-    fr.hent[b + r] = make_float4(p.x, p.y, __int_as_float(i), 0.f);
+    fr.hent[cell_start(fr, key >= 0 ? key : fr.ncell) + fr.hslot[i]] = make_float4(p.x, p.y, __int_as_float(i), 0.f);
 }
 
 // AgentsMover.h:96-126 over the 3x3 (generally (2k+1)^2) stencil, Jacobi, the
@@ -597,6 +597,8 @@ __global__ void k_hash_rank(DevAgents ag, DevFrame fr) {
 // position in place and the sum still sees only post-move positions.
 constexpr int kAvoidThreads = 256;
 constexpr int kAvoidList = 16;  // neighbours an agent keeps in LDS; more are taken by rescans
+constexpr int kAvoidRowsHalf = 2;  // stencils of up to 2 * 2 + 1 rows read their rows at once
+constexpr int kAvoidRowLoad = 8;   // entries of each row requested at once
 
 __device__ __forceinline__ void avoid_term(float2 p, float2 o, const CrowdParams& cp, float& sx, float& sy,
                                            int& cnt) {
@@ -619,44 +621,109 @@ __global__ __launch_bounds__(kAvoidThreads) void k_avoid_progress(DevMap map, De
     __shared__ int lj[kAvoidList * kAvoidThreads];
     __shared__ float2 lp[kAvoidList * kAvoidThreads];
     const int t = threadIdx.x;
-    const int i = blockIdx.x * blockDim.x + t;
-    if (i >= ag.n || fr.counters[5] == 0) return;
-    const float2 p = ag.pos[i];
+    // thread q takes the agent of hash entry q: the threads of a wave take
+    // agents of neighbouring cells and read the same stencil entries
+    const int q_self = blockIdx.x * blockDim.x + t;
+    if (q_self >= ag.n || fr.counters[5] == 0) return;
+    const float4 me = fr.hent[q_self];
+    const int i = __float_as_int(me.z);
+    const float2 p = make_float2(me.x, me.y);
+    // What does not depend on the avoidance is requested now, beside it: the
+    // path progress inputs (the triangle of the agent's next corridor node,
+    // the corridor node after it and its centroid) and the triangle of the
+    // agent's current node (the first test of whether the avoided position is
+    // on the navmesh).
+    const int moving = ag.in_m[i];
+    const int idx0 = ag.step_idx[i];
+    const int node = ag.step_node[i];
+    const int plen = ag.path_len[i];
+    const int cur = ag.curr[i];
+    const int tn = moving && idx0 != -1 ? node : 0;
+    const float2 ta = map.tri[3 * (size_t)tn], tb = map.tri[3 * (size_t)tn + 1], tc = map.tri[3 * (size_t)tn + 2];
+    const int cn = cur >= 0 ? cur : 0;
+    const float2 ua = map.tri[3 * (size_t)cn], ub = map.tri[3 * (size_t)cn + 1], uc = map.tri[3 * (size_t)cn + 2];
+    // corridor index the progress reads: 0 when the walk starts, else the one
+    // after the current step (read only when it exists)
+    const int nidx = idx0 == -1 ? 0 : idx0 + 1;
+    const bool pread = moving && (idx0 == -1 || nidx < plen);
+    const int* const slab = *ag.slab_now;
+    const int pn = *(pread ? slab + ag.path_off[i] + nidx : slab);
+    const float2 pc = map.cen[pread ? pn : 0];
     float sx = 0.f, sy = 0.f;
     int cnt = 0;
-    const int key = fr.hkey[i];
-    if (key >= 0) {
-        const int cx = key % map.hnx, cy = key / map.hnx;
+    if (q_self < fr.hstart[fr.ncell]) {  // a hashed agent (entries past them: NaN positions)
+        const int cx = cell_of(p.x, map.hx0, map.hinv, map.hnx);
+        const int cy = cell_of(p.y, map.hy0, map.hinv, map.hny);
+        const int key = cy * map.hnx + cx;
         const int x0 = max(0, cx - map.hk), x1 = min(map.hnx - 1, cx + map.hk);
         const int y0 = max(0, cy - map.hk), y1 = min(map.hny - 1, cy + map.hk);
-        const float4 self = make_float4(p.x, p.y, __int_as_float(i), 0.f);
-        int k = 0;
-        for (int yy = y0; yy <= y1; ++yy) {
-            const int row = yy * map.hnx;
-            const int b = fr.hstart[row + x0], e = fr.hstart[row + x1 + 1];
-            // the row's candidates four at a time, so their loads go out
-            // together (a short group is padded with the agent itself, which
-            // is skipped); the list order does not matter, the sum below
-            // takes the neighbours by ascending index
-            for (int q0 = b; q0 < e; q0 += 4) {
+        // k: neighbours found (the first kAvoidList go to the LDS list); r: the
+        // agent's rank in its cell, the number of the cell's agents with a
+        // smaller index (its place in the stable counting sort's order)
+        int k = 0, r = 0;
+        const int ob = cell_start(fr, key), oe = cell_start(fr, key + 1);
+        // entry q of the hash (q = -1: a padding entry, which holds the agent
+        // itself and counts for nothing)
+        auto take = [&](int q, const float4& o) {
+            const int j = __float_as_int(o.z);
+            r += (q >= ob && q < oe && j < i) ? 1 : 0;
+            if (j == i) return;
+            const float dx = p.x - o.x, dy = p.y - o.y;
+            if (dx * dx + dy * dy < cp.cd2) {
+                if (k < kAvoidList) {
+                    lj[k * kAvoidThreads + t] = j;
+                    lp[k * kAvoidThreads + t] = make_float2(o.x, o.y);
+                }
+                ++k;
+            }
+        };
+        // a row's entries from q0 on, four at a time, so their loads go out
+        // together; the list order does not matter, the sum below takes the
+        // neighbours by ascending index
+        auto take_rest = [&](int q0, int e) {
+            for (; q0 < e; q0 += 4) {
                 float4 ee[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) ee[u] = q0 + u < e ? fr.hent[q0 + u] : self;
+                for (int u = 0; u < 4; ++u) ee[u] = fr.hent[q0 + u < e ? q0 + u : q_self];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int j = __float_as_int(ee[u].z);
-                    if (j == i) continue;
-                    float dx = p.x - ee[u].x, dy = p.y - ee[u].y;
-                    if (dx * dx + dy * dy < cp.cd2) {
-                        if (k < kAvoidList) {
-                            lj[k * kAvoidThreads + t] = j;
-                            lp[k * kAvoidThreads + t] = make_float2(ee[u].x, ee[u].y);
-                        }
-                        ++k;
-                    }
+                for (int u = 0; u < 4; ++u) take(q0 + u < e ? q0 + u : -1, ee[u]);
+            }
+        };
+        if (map.hk <= kAvoidRowsHalf) {
+            // every row's bounds, then the first kAvoidRowLoad entries of
+            // every row, each group requested at once: two waits for memory
+            // where a row-by-row walk waits twice per row
+            int rb[2 * kAvoidRowsHalf + 1], re[2 * kAvoidRowsHalf + 1];
+#pragma unroll
+            for (int rr = 0; rr <= 2 * kAvoidRowsHalf; ++rr) {
+                const bool v = y0 + rr <= y1;
+                const int row = (v ? y0 + rr : y0) * map.hnx;
+                const int b = cell_start(fr, row + x0), e = cell_start(fr, row + x1 + 1);
+                rb[rr] = v ? b : 0;
+                re[rr] = v ? e : 0;
+            }
+            float4 ent[2 * kAvoidRowsHalf + 1][kAvoidRowLoad];
+#pragma unroll
+            for (int rr = 0; rr <= 2 * kAvoidRowsHalf; ++rr)
+#pragma unroll
+                for (int u = 0; u < kAvoidRowLoad; ++u) {
+                    const int q = rb[rr] + u;
+                    ent[rr][u] = fr.hent[q < re[rr] ? q : q_self];
                 }
+#pragma unroll
+            for (int rr = 0; rr <= 2 * kAvoidRowsHalf; ++rr)
+#pragma unroll
+                for (int u = 0; u < kAvoidRowLoad; ++u) take(rb[rr] + u < re[rr] ? rb[rr] + u : -1, ent[rr][u]);
+            // rows longer than that
+#pragma unroll
+            for (int rr = 0; rr <= 2 * kAvoidRowsHalf; ++rr) take_rest(rb[rr] + kAvoidRowLoad, re[rr]);
+        } else {
+            for (int yy = y0; yy <= y1; ++yy) {
+                const int row = yy * map.hnx;
+                take_rest(cell_start(fr, row + x0), cell_start(fr, row + x1 + 1));
             }
         }
+        fr.horder[ob + r] = i;
         if (k <= kAvoidList) {
             int last = -1;
             for (int m = 0; m < k; ++m) {
@@ -680,7 +747,7 @@ __global__ __launch_bounds__(kAvoidThreads) void k_avoid_progress(DevMap map, De
             float2 bp = make_float2(0.f, 0.f);
             for (int yy = y0; yy <= y1; ++yy) {
                 const int row = yy * map.hnx;
-                const int b = fr.hstart[row + x0], e = fr.hstart[row + x1 + 1];
+                const int b = cell_start(fr, row + x0), e = cell_start(fr, row + x1 + 1);
                 for (int q = b; q < e; ++q) {
                     const float4 o = fr.hent[q];
                     const int j = __float_as_int(o.z);
@@ -704,25 +771,25 @@ __global__ __launch_bounds__(kAvoidThreads) void k_avoid_progress(DevMap map, De
         const float rx = ax * cp.rot_cs - ay * cp.rot_sn;
         const float ry = ax * cp.rot_sn + ay * cp.rot_cs;
         const float nx = p.x + rx, ny = p.y + ry;
-        if (locate(map, nx, ny) >= 0) out = make_float2(nx, ny);
+        // on the navmesh when some triangle holds it: the current node's and
+        // the next corridor node's are tried before the point location
+        const bool on = (cur >= 0 && in_tri_v(ua, ub, uc, nx, ny)) ||
+                        (moving && idx0 != -1 && in_tri_v(ta, tb, tc, nx, ny)) || locate(map, nx, ny) >= 0;
+        if (on) out = make_float2(nx, ny);
     }
     ag.pos[i] = out;
-    fr.h_pos_out[i] = out;
     // AgentsMover.h:137-176 (after the Jacobi write-back)
-    if (!ag.in_m[i]) return;
-    const int* path = *ag.slab_now + ag.path_off[i];
-    int idx = ag.step_idx[i];
+    if (!moving) return;
+    int idx = idx0;
     if (idx == -1) {
-        const int n = path[0];
         ag.step_idx[i] = 0;
-        ag.step_node[i] = n;
-        ag.step_pos[i] = map.cen[n];
+        ag.step_node[i] = pn;  // corridor node 0
+        ag.step_pos[i] = pc;
         return;
     }
-    const int node = ag.step_node[i];
-    if (!in_tri(map, node, out.x, out.y)) return;
+    if (!in_tri_v(ta, tb, tc, out.x, out.y)) return;
     ag.curr[i] = node;
-    const int count = ag.path_len[i];
+    const int count = plen;
     if (idx < count) idx += 1;
     ag.step_idx[i] = idx;
     if (idx == count) {
@@ -733,10 +800,17 @@ __global__ __launch_bounds__(kAvoidThreads) void k_avoid_progress(DevMap map, De
         else
             ag.state[i] = kIdle;
     } else {
-        const int n = path[idx];
-        ag.step_node[i] = n;
-        ag.step_pos[i] = map.cen[n];
+        ag.step_node[i] = pn;  // corridor node idx = idx0 + 1
+        ag.step_pos[i] = pc;
     }
+}
+
+// The frame's positions into pinned host memory in agent order (the host
+// writes stay whole lines), after the avoidance pass has written them all.
+__global__ void k_pos_out(DevAgents ag, DevFrame fr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ag.n || fr.counters[5] == 0) return;
+    fr.h_pos_out[i] = ag.pos[i];
 }
 
 // Staging for bulk point location without allocation: the points, then the
@@ -803,17 +877,17 @@ hipError_t launch_compact(const DevAgents& ag, const DevSlots& sl, const DevFram
 
 hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& fr,
                         const CrowdParams& cp, hipStream_t st) {
-    hipError_t e;
     const int nb = nblk(ag.n, 256);
     hipLaunchKernelGGL(k_move_key, dim3(nb), dim3(256), 0, st, map, ag, fr);
     // the scan reads and zeroes the counts; nothing in a frame with no mover
-    if ((e = exscan(fr.hcount, fr.ncell, fr.hblk, fr.hstart, &fr.hstart[fr.ncell], st, fr.hcount,
+    hipError_t e;
+    if ((e = exscan(fr.hcount, fr.ncell + 1, fr.hblk, fr.hstart, &fr.hstart[fr.ncell + 1], st, fr.hcount,
                     &fr.counters[5])) != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_hash_place, dim3(nb), dim3(256), 0, st, ag, fr);
-    hipLaunchKernelGGL(k_hash_rank, dim3(nb), dim3(256), 0, st, ag, fr);
     hipLaunchKernelGGL(k_avoid_progress, dim3(nblk(ag.n, kAvoidThreads)), dim3(kAvoidThreads), 0, st, map, ag, fr,
                        cp);
+    hipLaunchKernelGGL(k_pos_out, dim3(nb), dim3(256), 0, st, ag, fr);
     return hipGetLastError();
 }
 
