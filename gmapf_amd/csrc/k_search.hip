@@ -384,6 +384,25 @@ __device__ __forceinline__ void park_walk(int l, int slot, int cbase, const DevS
     }
 }
 
+// Per-lane flags carried from one round to the next, kept as the bits of one
+// word (a vector register).  As separate booleans each would live across the
+// whole round as a 64-bit lane mask in scalar registers, which the round's
+// compares and selects need.
+enum : unsigned {
+    kHasReq = 1u,       // the segment holds a search or a corridor walk
+    kNoMore = 2u,       // the segment has no more work to take
+    kTookPlain = 4u,    // the segment took its one item (one search per segment)
+    kNeedPop = 8u,      // the lane pops its first node at the top of the round
+    kPoppedNext = 16u,  // the lane has a node to expand in the coming round
+    kWalking = 32u,     // batched form: the segment walks a found search's corridor
+    kPfVnOk = 64u,      // the prefetched marks belong to the coming node
+};
+struct LaneFlags {
+    unsigned v = 0u;
+    __device__ __forceinline__ bool get(unsigned b) const { return (v & b) != 0u; }
+    __device__ __forceinline__ void set(unsigned b, bool on) { v = on ? (v | b) : (v & ~b); }
+};
+
 // Resident waves per SIMD the launch needs to run one search per segment at
 // 16,384 searches: T/4 (4 at T = 16: at most 128 VGPRs; 2 at T <= 8: at most
 // 256).
@@ -434,11 +453,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
 
     // per-segment search state (replicated in every lane of the segment)
     int slot = -1, agent = -1;
-    bool has_req = false, no_more = sub >= sp.seg, took_plain = false;
+    LaneFlags fl;
+    fl.set(kNoMore, sub >= sp.seg);
     // the node this lane expands in the coming round: taken from the heap
     // and the items dealt to the lane at the end of the previous round, or
     // popped at the top of the first round of a fresh search
-    bool need_pop = false, popped_next = false;
     int c_next = -1;
     int S = 0, Tg = 0, cnt = 0;
     unsigned mval = 0;  // this search's mark generation, 1..127
@@ -447,14 +466,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     // corridor walk, for the budget), srch_here its search rounds
     int rounds_here = 0, srch_here = 0, exp_here = 0, disc_here = 0;
     // batched form: the segment walks its search's corridor (cbase, clen)
-    bool walking = false;
     int cbase = 0, clen = 0;
     int status = 0;
     uint2* vis = slots.vis;
     unsigned char* mark = slots.mark;
     // next round's pop, fetched while this round's frontier is pushed
     int pf_c = -1;
-    bool pf_vn_ok = false;
     int4 pf_r0 = make_int4(-1, -1, -1, -1), pf_r1 = make_int4(0, 0, 0, 0);
     int2 pf_r2 = make_int2(0, 0);
     uint2 pf_vc = make_uint2(0u, 0u);
@@ -469,22 +486,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         // test when every segment is busy; rare paths of the round are
         // skipped the same way, so a round pays no lane-mask bookkeeping for
         // them)
-        if (__ballot(!has_req && !no_more) != 0ull) {
-            if (!has_req && !no_more) {
+        if (__ballot(!fl.get(kHasReq) && !fl.get(kNoMore)) != 0ull) {
+            if (!fl.get(kHasReq) && !fl.get(kNoMore)) {
                 int item = -1;
                 if (persistent && !one_each) {
                     if (l == 0) item = atomicAdd(&fr.counters[2], 1);
                     item = __shfl(item, sub * T, kWave);
-                } else if (!took_plain) {
+                } else if (!fl.get(kTookPlain)) {
                     // every search has a segment of its own: segment i takes
                     // item i, once (the plain form always does)
                     item = blockIdx.x * sp.seg + sub;
-                    took_plain = true;
+                    fl.set(kTookPlain, true);
                 }
                 // slot = agent: nothing to do when no search is in flight there
                 if (sp.ident > 0 && item >= 0 && item < n_active && slots.agent[item] < 0) item = -1;
                 if (item < 0 || item >= n_active) {
-                    no_more = true;
+                    fl.set(kNoMore, true);
                 } else {
                     slot = sp.ident > 0 ? item : fr.active[(int)(((long long)item * stride) % n_active)];
                     agent = slots.agent[slot];
@@ -497,24 +514,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                     cT = map.cen[Tg];
                     rounds_here = 0;
                     srch_here = 0;
-                    walking = false;
+                    fl.set(kWalking, false);
                     exp_here = 0;
                     disc_here = 0;
                     vis = slots.vis + (size_t)slot * V;
                     mark = slots.mark + (size_t)slot * slots.VP;
                     pf_c = -1;
-                    pf_vn_ok = false;
-                    need_pop = status == 3;
-                    popped_next = false;
-                    has_req = true;
+                    fl.set(kPfVnOk, false);
+                    fl.set(kNeedPop, status == 3);
+                    fl.set(kPoppedNext, false);
+                    fl.set(kHasReq, true);
                     if (status == 4) {
                         // found earlier, corridor deferred for lack of slab space
                         const bool hold = sp.budget > 0;
                         const bool ok = start_corridor(l, slot, agent, S, Tg, slots.mS[slot], slots.mT[slot], vis, slab,
                                                        ag, slots, V, &wshare[3 * sub], walk, hold, cbase, clen, st_found,
                                                        st_corr);
-                        walking = hold && ok;
-                        has_req = walking;
+                        fl.set(kWalking, hold && ok);
+                        fl.set(kHasReq, fl.get(kWalking));
                         status = 0;
                     } else if (status == 5) {
                         // batched form: a corridor walk parked at the end of the
@@ -532,7 +549,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                             walk.idx = cbase + (l == 0 ? pw.y : pw.w);
                             if (walk.active) walk.ent = vis[n];
                         }
-                        walking = true;
+                        fl.set(kWalking, true);
                         status = 0;
                     } else if (status == 3) {
                         // Stamps 127e .. 127e + 126 (epoch e) use generations
@@ -574,7 +591,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                         cnt = slots.qcnt[qb];
                         const unsigned long long* src = slots.qsave + qb * (size_t)Q;
                         const unsigned long long k0 = src[0];
-                        popped_next = k0 != kKeyNone;
+                        fl.set(kPoppedNext, k0 != kKeyNone);
                         c_next = key_node(k0);
                         for (int i0 = 0; i0 < Q - 1; i0 += 8) {
                             unsigned long long x[8];
@@ -595,12 +612,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         // every segment's tickets are spent.  (A loop around the acquisition
         // would do the same without the idle round, but its code made every
         // round 9% slower.)
-        if (__ballot(has_req || !no_more) == 0ull) break;
+        if (__ballot(fl.get(kHasReq) || !fl.get(kNoMore)) == 0ull) break;
         wave_sync();  // seeds, restored queues and visited writes visible
 
         // ---- pop (first round of a fresh search) + neighbour fetch
-        if (__ballot(has_req && need_pop) != 0ull) {
-            const bool on = has_req && need_pop && cnt > 0;
+        if (__ballot(fl.get(kHasReq) && fl.get(kNeedPop)) != 0ull) {
+            const bool on = fl.get(kHasReq) && fl.get(kNeedPop) && cnt > 0;
             if (on) {
                 c_next = key_node(q[L]);
                 --cnt;
@@ -609,13 +626,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             const unsigned long long last = q[lrow * kWave + L];
             q[lrow * kWave + L] = kKeyNone;  // the vacated entry
             heap_sift_down(q, L, Q, depth, on && cnt > 0, last);
-            if (has_req && need_pop) popped_next = on;
-            need_pop = false;
+            if (fl.get(kHasReq) && fl.get(kNeedPop)) fl.set(kPoppedNext, on);
+            fl.set(kNeedPop, false);
         }
-        const bool popped = has_req && popped_next;
+        const bool popped = fl.get(kHasReq) && fl.get(kPoppedNext);
         const int c = popped ? c_next : -1;
         const bool miss = popped && c != pf_c;  // first round of a search: nothing prefetched
-        const bool have_vn = popped && pf_vn_ok && !miss;
+        const bool have_vn = popped && fl.get(kPfVnOk) && !miss;
         int4 r0 = pf_r0, r1 = pf_r1;
         int2 r2 = pf_r2;
         uint2 vc = pf_vc;
@@ -763,7 +780,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         const int ms = meet[0] ? 0 : (meet[1] ? 1 : (meet[2] ? 2 : 3));
         const unsigned long long mball = __ballot(ms < 3);
         const unsigned long long mb = mball & smask;
-        const bool found = has_req && !walking && (mb != 0ull);
+        const bool found = fl.get(kHasReq) && !fl.get(kWalking) && (mb != 0ull);
         int mS = -1, mT = -1;
         if (mball != 0ull) {
             const int w = found ? __ffsll((long long)mb) - 1 : sub * T;
@@ -796,7 +813,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         // since every round ends with a pop).  When m is the top, the first
         // dealt item takes its place (one sift down instead of a push and a
         // pop).
-        const bool cont = has_req && !walking && !found;
+        const bool cont = fl.get(kHasReq) && !fl.get(kWalking) && !found;
         unsigned long long d0 = stage[segbase + l], d1 = stage[segbase + l + T], d2 = stage[segbase + l + 2 * T];
         d0 = cont && l < tot ? d0 : kKeyNone;
         d1 = cont && l + T < tot ? d1 : kKeyNone;
@@ -845,7 +862,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                 pf_vn[0] = pmark[a0 >= 0 ? a0 : 0];
                 pf_vn[1] = pmark[a1 >= 0 ? a1 : 0];
                 pf_vn[2] = pmark[a2 >= 0 ? a2 : 0];
-                pf_vn_ok = has_next;
+                fl.set(kPfVnOk, has_next);
             }
             // the items still to push, at most two: the dealt items other
             // than m (and d0 when it replaced the root)
@@ -855,21 +872,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             heap_push(q, L, Q, depth, Q - 1, cnt, pa == m ? kKeyNone : pa);
             heap_push(q, L, Q, depth, Q - 1, cnt, pb == m ? kKeyNone : pb);
         }
-        popped_next = has_next;
+        fl.set(kPoppedNext, has_next);
         c_next = pf_c;
         const unsigned long long nb = __ballot(has_next) & smask;
-        const bool searching = has_req && !walking;
-        rounds_here += has_req ? 1 : 0;
+        const bool searching = fl.get(kHasReq) && !fl.get(kWalking);
+        rounds_here += fl.get(kHasReq) ? 1 : 0;
         srch_here += searching ? 1 : 0;
         exp_here += searching ? __popcll(pb) : 0;
         disc_here += searching ? tot : 0;
         status = !searching ? status : (found ? 1 : (nb == 0ull ? 2 : status));
         // batched form: a walking segment is done when both of its sides are
-        const bool wdone = walking && (__ballot(walk.active) & smask) == 0ull;
+        const bool wdone = fl.get(kWalking) && (__ballot(walk.active) & smask) == 0ull;
         const bool ended = (searching && status != 0) || wdone;
-        const bool park = has_req && !ended && sp.budget > 0 && rounds_here >= sp.budget;
+        const bool park = fl.get(kHasReq) && !ended && sp.budget > 0 && rounds_here >= sp.budget;
         if (__ballot(ended || park) != 0ull && (ended || park)) {
-            if (park && !walking) {
+            if (park && !fl.get(kWalking)) {
                 // [node to expand next or kKeyNone, heap items]: at most Q keys
                 const size_t qb = ((size_t)slot * T + l);
                 slots.qcnt[qb] = cnt;
@@ -877,7 +894,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                 dst[0] = m;
                 for (int i = 0; i < cnt; ++i) dst[1 + i] = q[i * kWave + L];
             }
-            if (park && walking) park_walk(l, slot, cbase, slots, walk);
+            if (park && fl.get(kWalking)) park_walk(l, slot, cbase, slots, walk);
             if (l == 0) {
                 if (searching) {
                     slots.status[slot] = status;
@@ -921,11 +938,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                     }
                 }
             }
-            walking = walk_on;
-            has_req = walk_on;
+            fl.set(kWalking, walk_on);
+            fl.set(kHasReq, walk_on);
             if (walk_on) {
                 status = 0;
-                popped_next = false;
+                fl.set(kPoppedNext, false);
             }
         }
     }
