@@ -158,6 +158,22 @@ def serving_main(args):
         log("serving", {k: v for k, v in r.items() if k != "stats"})
         runs.append(r)
     top = runs[-1]
+    # roofline of the search launches: a separate run at the top rate with HIP
+    # events around every map's search launch in every 8th frame (their graph
+    # nodes lengthen a frame, so the reported runs carry none); the largest
+    # map's launch is the dominant kernel
+    tr = run_serving(smaps, top["lam"], steps, warmup=args.warmup, T=T, Q=Q, round_budget=B,
+                     timing=SEARCH_TIMING_EVERY)
+    per_map = [dict(gbs=round(t["bytes"] / (t["ms"] / 1e3) / 1e9, 2) if t["ms"] > 0 else None,
+                    ms_per_launch=round(t["ms"] / t["launches"], 4) if t["launches"] else None,
+                    bytes_per_launch=round(t["bytes"] / t["launches"], 1) if t["launches"] else None)
+               for t in tr["search_timed"]]
+    big = tr["search_timed"][-1]
+    gbs = big["bytes"] / (big["ms"] / 1e3) / 1e9 if big["ms"] > 0 else 0.0
+    roof = dict(bound="hbm", achieved=round(gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(gbs / HBM_PEAK_GBS, 5), traffic=None)
+    log("serving search roofline per map", per_map)
+    cpu = None if args.skip_cpu else serving_cpu_baseline(smaps, loc, T, Q)
     out = {
         "metric": "serving loop: path solves/s with p50/p99 frame ms against a 16.6 ms frame budget",
         "value": round(top["solves_per_s"], 1),
@@ -179,15 +195,49 @@ def serving_main(args):
             "T": T, "Q": Q, "round_budget": B,
             "footprint_bytes_per_request": top["footprint_bytes"],
             "runs": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()
-                      if k not in ("stats", "nodes", "footprint_bytes")} | {
+                      if k not in ("stats", "nodes", "footprint_bytes", "search_timed")} | {
                          "admitted": r["stats"]["admitted"], "pending_at_end": r["stats"]["pending"],
                          "peak_live_bytes": r["stats"]["peak_live_bytes"]} for r in runs],
+            "search_launch_per_map": per_map,
             "parallelism": "single GPU, maps on separate streams",
         },
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def serving_cpu_baseline(smaps, loc, T, Q, seconds=12.0):
+    """Single-core oracle on a bounded sample of the serving request mix: a
+    uniformly chosen map, one of its agents' start nodes and a target node from
+    the map's pool (the same draw as the serving runs), searched until
+    ~`seconds` of CPU time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    oms, starts, pools = [], [], []
+    for k, sm in enumerate(smaps):
+        g = loc.keep[k]
+        tri, nbr, _ = g.map_arrays()
+        oms.append(pyoracle.OracleMap(tri, nbr))
+        starts.append(g.locate(sm.starts))
+        pools.append(g.locate(sm.targets))
+    rng = np.random.Generator(np.random.PCG64(11))
+    found = n = 0
+    t0 = time.process_time()
+    while time.process_time() - t0 < seconds:
+        m = int(rng.integers(0, len(smaps)))
+        s_node = int(starts[m][int(rng.integers(0, len(starts[m])))])
+        t_node = int(pools[m][int(rng.integers(0, len(pools[m])))])
+        if s_node == t_node:
+            continue
+        ok, _, _ = oms[m].search(s_node, t_node, T, Q, 1.0)
+        found += int(ok)
+        n += 1
+    el = time.process_time() - t0
+    return dict(value=round(found / el, 2), unit="solves/s", cores=1, kind="port",
+                sample=f"{n} searches of the serving request mix (uniform map, start node of a uniform agent, "
+                       f"target node from the map's pool), single-threaded oracle, {el:.1f} s CPU", **host_cpu())
 
 
 def crowd_bytes(g, cell, n_moving):

@@ -165,12 +165,13 @@ def serving_maps(locate_factory, seed=1, sizes=(256.0, 1024.0, 4096.0), agents=(
     return out
 
 
-def build_map(sm: ServingMap, T=8, Q=32, round_budget=256, mode=MODE_BATCHED, agents_max=32768, K=1.0):
+def build_map(sm: ServingMap, T=8, Q=32, round_budget=256, mode=MODE_BATCHED, agents_max=32768, K=1.0,
+              timing=0):
     g = GMAPF()
     g.init(Config(agents_max_count=max(agents_max, sm.n_agents), agent_radius=2.0, agent_speed=1.0,
                   path_finder_parallel_agents=sm.P, path_finder_threads_per_agents=T,
                   path_finder_queue_capacity=Q, path_finder_heuristic_k=K, step_mode=mode,
-                  round_budget=round_budget))
+                  round_budget=round_budget, search_timing=timing))
     g.fill_map_start(sm.size, sm.size)
     for o in sm.obstacles:
         g.fill_map_obstacle(o)
@@ -182,12 +183,17 @@ def build_map(sm: ServingMap, T=8, Q=32, round_budget=256, mode=MODE_BATCHED, ag
 
 
 def run_serving(smaps, lam, frames, warmup=10, dt=1.0 / 60, seed=7, T=8, Q=32, round_budget=256,
-                queue_memory_budget=2 << 30, log=None):
+                queue_memory_budget=2 << 30, log=None, timing=0):
     """Poisson(lam * dt) re-path arrivals per frame, each for a uniformly
     chosen map, a uniformly chosen agent of it and a target from the map's
     free-space pool; returns frame-latency percentiles, solves/s and request
-    latencies over the measured frames."""
-    engines = [build_map(sm, T, Q, round_budget) for sm in smaps]
+    latencies over the measured frames.  timing = k > 0: HIP events around
+    every map's search launch in every k-th frame, and per map the
+    algorithmic bytes (48 per expansion, 16 per discovery, 8 per corridor
+    node) and durations of those launches ("search_timed")."""
+    engines = [build_map(sm, T, Q, round_budget, timing=timing) for sm in smaps]
+    timed = [[0.0, 0.0, 0] for _ in engines]  # bytes, ms, launches
+    eprev = [e.stats() for e in engines] if timing else None
     srv = Server(engines, queue_memory_budget=queue_memory_budget)
     rng = np.random.Generator(np.random.PCG64(seed))
     n_ag = np.array([sm.n_agents for sm in smaps], dtype=np.int64)
@@ -205,6 +211,16 @@ def run_serving(smaps, lam, frames, warmup=10, dt=1.0 / 60, seed=7, T=8, Q=32, r
         srv.step(dt)
         st = srv.stats()
         done = srv.poll()
+        if timing:
+            for m, e in enumerate(engines):
+                es, ems = e.stats(), e.last_frame_ms()
+                if f >= warmup and ems[2] >= 0:
+                    timed[m][0] += 48.0 * (es["expansions"] - eprev[m]["expansions"]) + 16.0 * (
+                        es["discoveries"] - eprev[m]["discoveries"]) + 8.0 * (
+                        es["corridor_nodes"] - eprev[m]["corridor_nodes"])
+                    timed[m][1] += float(ems[2])
+                    timed[m][2] += 1
+                eprev[m] = es
         if f >= warmup:
             ms = srv.last_frame_ms()
             frame_ms.append(ms[0])
@@ -230,4 +246,5 @@ def run_serving(smaps, lam, frames, warmup=10, dt=1.0 / 60, seed=7, T=8, Q=32, r
         request_latency_p50_ms=float(np.percentile(lat, 50)) if len(lat) else None,
         request_latency_p99_ms=float(np.percentile(lat, 99)) if len(lat) else None,
         host_admit_launch_wait_retire_ms=[round(float(x), 4) for x in np.mean(np.array(parts), axis=0)],
-        stats=final, footprint_bytes=foot, nodes=[i["V"] for i in info])
+        stats=final, footprint_bytes=foot, nodes=[i["V"] for i in info],
+        search_timed=[dict(bytes=b, ms=t, launches=n) for b, t, n in timed] if timing else None)
