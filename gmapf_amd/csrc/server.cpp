@@ -72,6 +72,7 @@ struct gmapf_server_s {
     Clock::time_point t_frame;
     double last_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<int> node_buf;
+    std::vector<size_t> launch_order;  // maps in the order their frames are launched
 
     void check(int rc, gmapf_handle h) {
         if (rc != GMAPF_OK) throw ServeError(rc, gmapf_last_error(h));
@@ -303,7 +304,17 @@ struct gmapf_server_s {
             pending.swap(keep);
         }
         const Clock::time_point t_adm = Clock::now();
-        for (gmapf_handle h : maps) check(gmapf_async_step(h, dt), h);
+        // The maps' frames run side by side on their own streams, and each
+        // launch costs tens of microseconds of host time, so the frame ends
+        // with the map whose search starts last.  The map with the most live
+        // search memory (the longest expected search work) is launched first;
+        // the order changes no result.
+        launch_order.resize(maps.size());
+        for (size_t m = 0; m < maps.size(); m++) launch_order[m] = m;
+        std::stable_sort(launch_order.begin(), launch_order.end(), [&](size_t a, size_t b) {
+            return (long long)live[a] * foot[a] > (long long)live[b] * foot[b];
+        });
+        for (size_t m : launch_order) check(gmapf_async_step(maps[m], dt), maps[m]);
         going = true;
         last_ms[1] = ms_between(t_frame, t_adm);
         last_ms[2] = ms_between(t_adm, Clock::now());
