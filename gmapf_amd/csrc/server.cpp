@@ -253,6 +253,7 @@ struct gmapf_server_s {
         adopt();
         std::vector<size_t> take;
         plan(take, &held_by_budget, true);
+        const Clock::time_point t_plan = Clock::now();
         // per map: ids and targets of this frame's admissions
         std::vector<std::vector<int>> ids(maps.size());
         std::vector<std::vector<float>> xy(maps.size());
@@ -264,37 +265,54 @@ struct gmapf_server_s {
             xy[p.map].push_back(p.y);
             which[p.map].push_back(k);
         }
+        // The maps' frames run side by side on their own streams, and each
+        // launch costs tens of microseconds of host time, so the frame ends
+        // with the map whose search starts last.  The map with the most
+        // search memory live after this frame's admissions (the longest
+        // expected search work) is admitted and launched first, each map's
+        // launch right after its own admissions; the order changes no result.
+        launch_order.resize(maps.size());
+        for (size_t m = 0; m < maps.size(); m++) launch_order[m] = m;
+        std::stable_sort(launch_order.begin(), launch_order.end(), [&](size_t a, size_t b) {
+            return (long long)(live[a] + ids[a].size()) * foot[a] > (long long)(live[b] + ids[b].size()) * foot[b];
+        });
         std::vector<unsigned char> gone(pending.size(), 0);
-        for (size_t m = 0; m < maps.size(); m++) {
-            if (ids[m].empty()) continue;
-            std::vector<int> nodes;
-            if (cfg.record_start_nodes) {
-                int n = agents_of((int)m);
-                node_buf.assign(n, -1);
-                check(gmapf_get_agents_soa(maps[m], nullptr, nullptr, node_buf.data()), maps[m]);
-            }
-            std::vector<int> st(ids[m].size(), 0);
-            check(gmapf_set_agent_targ_pos_many(maps[m], (int)ids[m].size(), ids[m].data(), xy[m].data(),
-                                                st.data()),
-                  maps[m]);
-            for (size_t j = 0; j < ids[m].size(); j++) {
-                const Pending& p = pending[which[m][j]];
-                gone[which[m][j]] = 1;
-                if (st[j] != GMAPF_OK) {  // cannot happen: targets were located at submit
-                    done.push_back(gmapf_request_done{p.ticket, p.map, p.agent, 0, -1, p.submit_frame, frame,
-                                                      frame, ms_between(p.t_submit, Clock::now())});
-                    n_completed++;
-                    n_nopath++;
-                    continue;
+        Clock::time_point t_adm = Clock::now();
+        double adm_ms = 0.0;
+        for (size_t m : launch_order) {
+            const Clock::time_point t_m = Clock::now();
+            if (!ids[m].empty()) {
+                if (cfg.record_start_nodes) {
+                    int n = agents_of((int)m);
+                    node_buf.assign(n, -1);
+                    check(gmapf_get_agents_soa(maps[m], nullptr, nullptr, node_buf.data()), maps[m]);
                 }
-                busy[m][p.agent] = 1;
-                live[m]++;
-                live_bytes += foot[m];
-                n_admitted++;
-                inflight.push_back(InFlight{p.ticket, (int)m, p.agent,
-                                            cfg.record_start_nodes ? node_buf[p.agent] : -1, p.submit_frame,
-                                            frame, p.t_submit});
+                std::vector<int> st(ids[m].size(), 0);
+                check(gmapf_set_agent_targ_pos_many(maps[m], (int)ids[m].size(), ids[m].data(), xy[m].data(),
+                                                    st.data()),
+                      maps[m]);
+                for (size_t j = 0; j < ids[m].size(); j++) {
+                    const Pending& p = pending[which[m][j]];
+                    gone[which[m][j]] = 1;
+                    if (st[j] != GMAPF_OK) {  // cannot happen: targets were located at submit
+                        done.push_back(gmapf_request_done{p.ticket, p.map, p.agent, 0, -1, p.submit_frame, frame,
+                                                          frame, ms_between(p.t_submit, Clock::now())});
+                        n_completed++;
+                        n_nopath++;
+                        continue;
+                    }
+                    busy[m][p.agent] = 1;
+                    live[m]++;
+                    live_bytes += foot[m];
+                    n_admitted++;
+                    inflight.push_back(InFlight{p.ticket, (int)m, p.agent,
+                                                cfg.record_start_nodes ? node_buf[p.agent] : -1, p.submit_frame,
+                                                frame, p.t_submit});
+                }
             }
+            t_adm = Clock::now();
+            adm_ms += ms_between(t_m, t_adm);
+            check(gmapf_async_step(maps[m], dt), maps[m]);
         }
         peak_bytes = std::max(peak_bytes, live_bytes);
         if (!take.empty()) {
@@ -303,21 +321,12 @@ struct gmapf_server_s {
                 if (!gone[i]) keep.push_back(pending[i]);
             pending.swap(keep);
         }
-        const Clock::time_point t_adm = Clock::now();
-        // The maps' frames run side by side on their own streams, and each
-        // launch costs tens of microseconds of host time, so the frame ends
-        // with the map whose search starts last.  The map with the most live
-        // search memory (the longest expected search work) is launched first;
-        // the order changes no result.
-        launch_order.resize(maps.size());
-        for (size_t m = 0; m < maps.size(); m++) launch_order[m] = m;
-        std::stable_sort(launch_order.begin(), launch_order.end(), [&](size_t a, size_t b) {
-            return (long long)live[a] * foot[a] > (long long)live[b] * foot[b];
-        });
-        for (size_t m : launch_order) check(gmapf_async_step(maps[m], dt), maps[m]);
         going = true;
-        last_ms[1] = ms_between(t_frame, t_adm);
-        last_ms[2] = ms_between(t_adm, Clock::now());
+        // host parts: [1] admission (planning and every map's admissions),
+        // [2] the launches
+        const double total = ms_between(t_frame, Clock::now());
+        last_ms[1] = ms_between(t_frame, t_plan) + adm_ms;
+        last_ms[2] = total - last_ms[1];
     }
 
     void wait_step_end() {
