@@ -76,10 +76,19 @@ struct gmapf_s {
     std::vector<float> pos;
     std::vector<int> state, curr, targ, gen;
     std::vector<float> targ_pos;
-    // target changes made after start, applied at the next AsyncStep
-    // at most one pending update per agent: upd_pos[id] is its index in
-    // upd_ids, or -1; a later SetAgentTargPos overwrites the entry in place
-    std::vector<int> upd_ids, upd_targ, upd_pos;
+    // Target changes made after start, applied at the next AsyncStep, at
+    // most one per agent (a later SetAgentTargPos overwrites the entry in
+    // place).  Between steps they are written straight into the next frame's
+    // pinned input (h_upd, h_upd_xy): entry upd_pos[id] holds agent id's
+    // change when upd_epoch_of[id] == upd_epoch, and upd_n entries are in
+    // use; a finished step starts a new epoch, so clearing is O(1).  Changes
+    // made while a step runs (its input is in use) queue in upd_ids / upd_targ
+    // / upd_xy (upd_qpos[id]: index, or -1) and move to the pinned input when
+    // the step ends.
+    std::vector<int> upd_ids, upd_targ, upd_qpos, upd_pos;
+    std::vector<unsigned> upd_epoch_of;
+    unsigned upd_epoch = 1;
+    int upd_n = 0;
     std::vector<float> upd_xy;
 
     // device
@@ -596,18 +605,7 @@ struct gmapf_s {
         // frame inputs go to pinned host memory that the frame's first
         // kernels read directly (no copies; the captured graph stays fixed)
         *h_step = cfg.agent_speed * dt;  // AgentsMover.h:299
-        h_upd[0] = 0;
-        if (!upd_ids.empty()) {
-            const int n = (int)upd_ids.size();  // one per agent (the last one made)
-            std::memcpy(h_upd + 1, upd_ids.data(), (size_t)n * sizeof(int));
-            std::memcpy(h_upd + 1 + upd_cap, upd_targ.data(), (size_t)n * sizeof(int));
-            std::memcpy(h_upd_xy, upd_xy.data(), (size_t)n * sizeof(float2));
-            h_upd[0] = n;
-            for (int id : upd_ids) upd_pos[id] = -1;
-            upd_ids.clear();
-            upd_targ.clear();
-            upd_xy.clear();
-        }
+        h_upd[0] = upd_n;  // the changes are already in place (stage_update)
         // compact the corridor slab before this frame's search when the
         // last frame left it half full (the host reads the slab top that the
         // last frame exported; no device round trip)
@@ -655,8 +653,19 @@ struct gmapf_s {
         compact_next = 2 * slab_in_use > dag.slab_cap;
         frames++;
         stats_host[7] = frames;
-        // target changes made while the step ran stay pending for the next step
-        for (int id : upd_ids) state[id] = GMAPF_STATE_SEARCH;
+        // the step has read its changes: a new epoch empties the pinned input;
+        // changes made while the step ran stay pending for the next step
+        upd_n = 0;
+        upd_epoch++;
+        for (size_t k = 0; k < upd_ids.size(); k++) {
+            const int id = upd_ids[k];
+            state[id] = GMAPF_STATE_SEARCH;
+            stage_update(id, upd_targ[k], upd_xy[2 * k], upd_xy[2 * k + 1]);
+            upd_qpos[id] = -1;
+        }
+        upd_ids.clear();
+        upd_targ.clear();
+        upd_xy.clear();
         const Clock::time_point t_end = Clock::now();
         last_ms[0] = std::chrono::duration<double, std::milli>(t_end - t_step).count();
         last_ms[5] = std::chrono::duration<double, std::milli>(t_enq - t_step).count();
@@ -753,6 +762,24 @@ struct gmapf_s {
         return n_agents++;
     }
 
+    // Agent id's target change into the next frame's pinned input (no step
+    // is running, so nothing reads it now).
+    void stage_update(int id, int node, float x, float y) {
+        if ((int)upd_pos.size() < n_agents) {
+            upd_pos.resize(n_agents, -1);
+            upd_epoch_of.resize(n_agents, 0u);
+        }
+        int k = upd_pos[id];
+        if (upd_epoch_of[id] != upd_epoch) {
+            k = upd_n++;
+            upd_pos[id] = k;
+            upd_epoch_of[id] = upd_epoch;
+            h_upd[1 + k] = id;
+        }
+        h_upd[1 + upd_cap + k] = node;
+        h_upd_xy[k] = make_float2(x, y);
+    }
+
     void set_target(int id, float x, float y) {
         require_init();
         if (!map_filled) throw ApiError(GMAPF_E_STATE, "map is not filled");
@@ -766,9 +793,11 @@ struct gmapf_s {
             targ_pos[2 * id] = x;
             targ_pos[2 * id + 1] = y;
             gen[id]++;
+        } else if (!step_going) {
+            stage_update(id, node, x, y);
         } else {
-            if ((int)upd_pos.size() < n_agents) upd_pos.assign(n_agents, -1);
-            int& k = upd_pos[id];
+            if ((int)upd_qpos.size() < n_agents) upd_qpos.resize(n_agents, -1);
+            int& k = upd_qpos[id];
             if (k >= 0) {  // an update of this agent is pending: the last one wins
                 upd_targ[k] = node;
                 upd_xy[2 * (size_t)k] = x;
