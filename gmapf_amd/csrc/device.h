@@ -119,7 +119,7 @@ struct DevFrame {
     int* active;     // P: active slot list for the search kernel
     int* counters;   // [0] n_requests [1] n_active [2] work ticket [3] n_ended
                      // [4] n_free [5] moving count [6] n_admitted
-                     // [7] blocks done in the admission count pass
+                     // [7] unused
                      // [8] unused
                      // [9] ticket stride: ticket t takes active[(t * stride) % n_active]
     unsigned long long* stats;  // [0] found [1] fail [2] expansions [3] discoveries

@@ -266,8 +266,7 @@ __device__ int admit_prefix(int* vals, int n, int* sh) {
 }
 
 __global__ __launch_bounds__(kAdmitThreads) void k_admit_count(DevAgents ag, DevSlots sl, DevFrame fr, int nbA) {
-    __shared__ int wsum_f[kAdmitWaves], wsum_m[kAdmitWaves], sh[kAdmitWaves];
-    __shared__ int is_last;
+    __shared__ int wsum_f[kAdmitWaves], wsum_m[kAdmitWaves];
     const int b = blockIdx.x;
     bool f = false, mv = false;
     if (b < nbA) {
@@ -297,13 +296,16 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_count(DevAgents ag, Dev
         } else {
             fr.sblk[b - nbA] = nf;
         }
-        __threadfence();
-        is_last = atomicAdd(&fr.counters[7], 1) == (int)gridDim.x - 1;
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    const int nbS = (int)gridDim.x - nbA;
+}
+
+// One block after k_admit_count: the blocks' prefixes, the frame's counters
+// and the ticket stride.  (A last-block-done tail inside k_admit_count would
+// save this launch, but every block's device-scope fence before the done
+// count writes its L2 lines back, which costs more than the launch on a
+// multi-XCD part.)
+__global__ __launch_bounds__(kAdmitThreads) void k_admit_finish(DevSlots sl, DevFrame fr, int nbA, int nbS) {
+    __shared__ int sh[kAdmitWaves];
     const int nreq = admit_prefix(fr.blk, nbA, sh);
     const int nfree = admit_prefix(fr.sblk, nbS, sh);
     int mv_part = 0;
@@ -319,7 +321,6 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_count(DevAgents ag, Dev
         fr.counters[4] = nfree;
         fr.counters[5] = moving;
         fr.counters[6] = admitted;
-        fr.counters[7] = 0;  // every block has counted itself: ready for the next frame
         // Searches still running from earlier frames (the long ones) come
         // first in the active list and new ones last; a stride coprime with
         // the count, near 0.618 of it, deals both kinds to every workgroup,
@@ -852,6 +853,7 @@ hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& 
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_admit_count, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
+    hipLaunchKernelGGL(k_admit_finish, dim3(1), dim3(kAdmitThreads), 0, st, sl, fr, nbA, nbS);
     hipLaunchKernelGGL(k_admit_scatter, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
     hipLaunchKernelGGL(k_admit_slots, dim3(nblk(sl.P, 256)), dim3(256), 0, st, ag, sl, fr);
     return hipGetLastError();
