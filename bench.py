@@ -555,11 +555,15 @@ def main():
     # so the timed frames also move, hash, steer and advance agents
     moving = None
     if not args.skip_moving and args.hold > 0:
-        mb = run_frames(sc, G.MODE_BATCHED, T, Q, B, args.warmup, args.steps, settle=settle, hold=args.hold,
-                        spw=spw)
-        mp = None if args.skip_plain else run_frames(sc, G.MODE_PLAIN, T, Q, 0, args.warmup, args.steps,
+        # agents re-path in waves every `hold` frames, so the window covers at
+        # least four of them whatever --steps is (a shorter one lands on a
+        # phase of the cycle, and the plain step's long search frames come
+        # once per cycle)
+        mk = max(args.steps, 4 * args.hold)
+        mb = run_frames(sc, G.MODE_BATCHED, T, Q, B, args.warmup, mk, settle=settle, hold=args.hold, spw=spw)
+        mp = None if args.skip_plain else run_frames(sc, G.MODE_PLAIN, T, Q, 0, args.warmup, mk,
                                                       settle=settle, hold=args.hold)
-        moving = {"hold_frames": args.hold, "moving_agents_mean": round(mb["moving_agents_mean"], 1),
+        moving = {"hold_frames": args.hold, "frames": mk, "moving_agents_mean": round(mb["moving_agents_mean"], 1),
                   "batched_solves_per_s": round(mb["solves_per_s"], 1), "batched_mean_ms": round(mb["mean_ms"], 4),
                   "plain_mean_ms": round(mp["mean_ms"], 4) if mp else None,
                   "plain_p50_ms": round(mp["p50_ms"], 4) if mp else None,
