@@ -13,7 +13,10 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("T,Q,spw", [(2, 16, 0), (4, 16, 0), (8, 32, 0), (16, 8, 0), (32, 4, 0), (64, 2, 0), (4, 1, 0),
-                                     (4, 16, 8), (2, 16, 7), (8, 16, 1)])
+                                     (4, 16, 8), (2, 16, 7), (8, 16, 1),
+                                     # the other compile-time queue kernels and the
+                                     # deeper heaps of the autotuner's grid
+                                     (2, 32, 0), (4, 32, 0), (4, 64, 0), (16, 128, 0), (64, 256, 0)])
 def test_single_frame_search_parity(oracle, T, Q, spw):
     """One frame, every agent searches once; corridors and flags vs
     oracle.search.  spw > 0: fewer searches per workgroup than 64 / T."""
