@@ -60,14 +60,18 @@ def _world(seed, nx, ny, n_agents, T, Q, mode=G.MODE_PLAIN, budget=0, p=0.2, P=N
     return tri, nbr, cfg, starts, targs
 
 
-@pytest.mark.parametrize("mode,budget,P,spw", [(G.MODE_PLAIN, 0, None, 0), (G.MODE_PLAIN, 0, 37, 0),
-                                               (G.MODE_BATCHED, 3, None, 0), (G.MODE_BATCHED, 1, 50, 0),
-                                               (G.MODE_BATCHED, 3, None, 3)])
-def test_multi_frame_world_parity(oracle, mode, budget, P, spw):
+@pytest.mark.parametrize("mode,budget,P,spw,T,Q", [(G.MODE_PLAIN, 0, None, 0, 8, 16), (G.MODE_PLAIN, 0, 37, 0, 8, 16),
+                                                   (G.MODE_BATCHED, 3, None, 0, 8, 16),
+                                                   (G.MODE_BATCHED, 1, 50, 0, 8, 16),
+                                                   (G.MODE_BATCHED, 3, None, 3, 8, 16),
+                                                   # queues parked and restored past 16 items
+                                                   (G.MODE_BATCHED, 2, None, 0, 4, 64),
+                                                   (G.MODE_BATCHED, 3, 60, 0, 2, 32)])
+def test_multi_frame_world_parity(oracle, mode, budget, P, spw, T, Q):
     """Agents search, move, avoid and progress for many frames; every frame
     must equal the oracle bit for bit (positions, states, nodes, corridors)."""
-    tri, nbr, cfg, starts, targs = _world(21, 40, 40, 160, 8, 16, mode, budget, P=P, spw=spw)
-    print(f"world parity mode={mode} budget={budget} P={P} spw={spw}", flush=True)
+    tri, nbr, cfg, starts, targs = _world(21, 40, 40, 160, T, Q, mode, budget, P=P, spw=spw)
+    print(f"world parity mode={mode} budget={budget} P={P} spw={spw} T={T} Q={Q}", flush=True)
     g = H.make_gpu(tri, nbr, cfg, starts, targs)
     w = H.make_oracle(oracle, tri, nbr, cfg, starts, targs)
     dt = 1.0 / 60
