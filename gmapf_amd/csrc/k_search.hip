@@ -26,7 +26,9 @@
 // node id) in LDS, entry i of lane L at q[i*64+L] (bank-conflict free);
 // when full, a smaller key replaces the largest leaf and sifts up.  Claim
 // prefix sums and the first-meet choice are wave ballots + popcounts;
-// duplicate claims inside a round are found by an LDS claim hash.  Visited
+// duplicate claims inside a round are found by DPP row shifts that compare
+// each candidate with the segment's earlier ones (T <= 4) or by an LDS claim
+// hash (T >= 8).  Visited
 // state is split by use: a per-slot byte map (m << 1 | dir, m = the search's
 // 7-bit generation, so maps are not cleared per search) answers the probe of
 // every neighbour with one byte, 64 nodes per cache line; an 8-byte entry
