@@ -22,7 +22,8 @@ Selection (``select``): two points are kept,
   * "fastest": the highest batched solves/s within --target-ms, whatever
     the ratio (when no point is within --target-ms: the lowest p99).
 The best `top` candidates of each are re-measured and the gates are applied to
-the re-measured figures before the choice.  Results are cached per
+the re-measured figures before the choice; re-measured rates within 1% of the
+best count as equal, and the lowest p99 frame among them wins.  Results are cached per
 (workload, agents) in gmapf_amd/tuned.json; the benchmark reports the
 "target" point (or "fastest" when no point reaches the ratio) and measures
 the other beside it.
@@ -109,10 +110,13 @@ def measure_shapes(sc, table, target_ms=4.0, best=3, warmup=5, steps=120, log=pr
     return rows
 
 
-def select(table, target_ms, ratio, confirm, top=3):
+def select(table, target_ms, ratio, confirm, top=3, tie=0.01):
     """Choose "target" and "fastest" from measured rows.  confirm(row)
     re-measures a candidate and returns (solves/s, p99 ms); the gates apply
-    to the re-measured figures.  Returns (target or None, fastest)."""
+    to the re-measured figures.  Re-measured rates within `tie` (relative) of
+    the best are taken as equal, and the lowest p99 frame among them wins (two
+    budgets that solve equally fast differ mostly in frame length).  Returns
+    (target or None, fastest)."""
 
     def pick(cands, need):
         ranked = sorted(cands, key=lambda r: -r["solves_per_s"])[:top]
@@ -121,7 +125,11 @@ def select(table, target_ms, ratio, confirm, top=3):
                 r["solves_confirm"], r["p99_confirm"] = confirm(r)
         ok = [r for r in ranked
               if r["p99_confirm"] <= target_ms and r["solves_confirm"] >= need * r["plain_solves_per_s"]]
-        return max(ok, key=lambda r: r["solves_confirm"]) if ok else None
+        if not ok:
+            return None
+        best = max(r["solves_confirm"] for r in ok)
+        equal = [r for r in ok if r["solves_confirm"] >= (1.0 - tie) * best]
+        return min(equal, key=lambda r: (r["p99_confirm"], -r["solves_confirm"]))
 
     within = [r for r in table if r["p99_ms"] <= target_ms]
     with_ratio = [r for r in within if r["ratio"] >= ratio]

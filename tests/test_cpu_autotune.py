@@ -1,7 +1,8 @@
 """The autotuner's selection rules (gmapf_amd/autotune.py select) on synthetic
 measurement tables: the p99 gate, the batched/plain ratio gate, re-measurement
 of the best few candidates before the gates, and block-shape rows competing
-with the default shape."""
+with the default shape, and the p99 frame deciding between equally fast
+points."""
 from gmapf_amd.autotune import _entry, select, shape_options
 
 
@@ -69,3 +70,24 @@ def test_block_shape_rows_compete_and_are_reported():
     target, fastest = select(table, 4.0, 5.0, confirm_from())
     assert target["spw"] == 8 and _entry(target)["spw"] == 8
     assert shape_options(4) == [8, 4] and shape_options(2) == [16, 8] and shape_options(64) == []
+
+
+def test_equal_rates_go_to_the_lower_p99():
+    # Re-measured within 1% of each other: the shorter p99 frame wins over the
+    # (slightly) faster point, for both picks.
+    table = [row(4, 16, 192, 11.854e6, 0.50, 1.25e6),
+             row(4, 16, 128, 11.840e6, 0.40, 1.25e6),
+             row(2, 16, 128, 11.52e6, 0.30, 1.25e6)]   # 2.8% slower: not a tie
+    confirm = confirm_from({(4, 16, 192, 0): (11.854e6, 0.53)})
+    target, fastest = select(table, 4.0, 5.0, confirm)
+    assert target["budget"] == 128 and target["T"] == 4
+    assert fastest["budget"] == 128 and fastest["T"] == 4
+
+
+def test_a_clear_rate_lead_beats_a_lower_p99():
+    table = [row(4, 16, 192, 12.0e6, 0.53, 1.25e6),    # 1.4% faster
+             row(4, 16, 128, 11.84e6, 0.40, 1.25e6)]
+    target, fastest = select(table, 4.0, 5.0, confirm_from())
+    assert target["budget"] == 192 and fastest["budget"] == 192
+    target, _ = select(table, 4.0, 5.0, confirm_from(), tie=0.02)
+    assert target["budget"] == 128
