@@ -43,6 +43,70 @@ def test_single_frame_search_parity(oracle, T, Q, spw):
     assert nfound > 0
 
 
+def _one_way_map(hub):
+    """A grid navmesh whose neighbour lists are no longer a mesh's (the
+    FillMapManually input may be any lists): the back link of about one edge
+    in seven dropped, and with `hub` one node appended to one list in twenty
+    of those with room, so that it has far more than four in-neighbours."""
+    tri, nbr = H.grid_map(11, 48, 40, cell=4.0, p=0.3)
+    nbr = np.array(nbr, dtype=np.int64)
+    r = S.stream(31, 3)
+    V = len(nbr)
+    for i in range(V):
+        for k in range(3):
+            j = nbr[i, k]
+            if j < 0:
+                break
+            if j > i and r.uniform() < 0.15:  # drop j -> i, keep the list compacted
+                row = [x for x in nbr[j] if x >= 0 and x != i]
+                nbr[j] = row + [-1] * (3 - len(row))
+    indeg = np.bincount(nbr[nbr >= 0], minlength=V)
+    if hub:
+        h = int(np.argmax(indeg))
+        for i in range(V):
+            row = [x for x in nbr[i] if x >= 0]
+            if len(row) < 3 and i != h and h not in row and r.uniform() < 0.05:
+                nbr[i] = row + [h] + [-1] * (2 - len(row))
+        indeg = np.bincount(nbr[nbr >= 0], minlength=V)
+        assert indeg.max() > 4
+    else:
+        assert indeg.max() <= 4
+    return tri, nbr
+
+
+@pytest.mark.parametrize("hub,mode,budget", [(False, G.MODE_PLAIN, 0), (True, G.MODE_PLAIN, 0),
+                                             (False, G.MODE_BATCHED, 3), (True, G.MODE_BATCHED, 3)])
+def test_world_parity_on_one_way_neighbour_lists(oracle, hub, mode, budget):
+    """Agents search and move over neighbour lists with one-way links (the
+    search follows a node's own list; a corridor walks parents, which need
+    not list their children), and with `hub` over a node listed by far more
+    than three nodes.  Every frame equals the oracle's (states, positions,
+    nodes; corridors every fifth frame); the batched form walks corridors
+    over several frames (parked walks)."""
+    tri, nbr = _one_way_map(hub)
+    cen = S.centroids(tri)
+    r = S.stream(6, 3)
+    V = len(tri)
+    perm = np.array(sorted(range(V), key=lambda _: r.next_u64()))
+    n = 200
+    starts = [cen[perm[i]] for i in range(n)]  # distinct start nodes
+    targs = [cen[r.randint(V)] for _ in range(n)]
+    cfg = G.Config(agents_max_count=n, path_finder_parallel_agents=n, path_finder_threads_per_agents=4,
+                   path_finder_queue_capacity=16, agent_radius=0.5, agent_speed=2.0, step_mode=mode,
+                   round_budget=max(1, budget))
+    print(f"one-way lists hub={hub} mode={mode} V={V}", flush=True)
+    g = H.make_gpu(tri, nbr, cfg, starts, targs)
+    w = H.make_oracle(oracle, tri, nbr, cfg, starts, targs)
+    for f in range(60):
+        g.step(1.0 / 60)
+        w.step(1.0 / 60)
+        H.compare_worlds(g, w, f, check_corridors=(f % 5 == 0))
+    gs, os_ = g.stats(), w.stats()
+    for k in ("found", "not_found", "expansions", "discoveries", "corridor_nodes", "rounds", "admitted"):
+        assert gs[k] == os_[k], (k, gs[k], os_[k])
+    assert gs["found"] > 50
+
+
 def _world(seed, nx, ny, n_agents, T, Q, mode=G.MODE_PLAIN, budget=0, p=0.2, P=None, spw=0):
     tri, nbr = H.grid_map(seed, nx, ny, cell=4.0, p=p)
     cen = S.centroids(tri)
