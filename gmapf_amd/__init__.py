@@ -64,6 +64,7 @@ class _Config(C.Structure):
         ("searches_per_wave", C.c_int),
         ("search_timing", C.c_int),
         ("search_workgroups", C.c_int),
+        ("search_wide_keys", C.c_int),
     ]
 
 
@@ -99,6 +100,7 @@ class Config:
     searches_per_wave: int = 0
     search_timing: int = 0  # HIP events around the search launch of every k-th frame; 0: none
     search_workgroups: int = 0  # batched form: cap on the persistent search grid; 0: automatic
+    search_wide_keys: bool = False  # keys without G and side also on maps of <= 32,768 nodes
 
     def _to_c(self) -> _Config:
         c = _Config()

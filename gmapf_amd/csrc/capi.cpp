@@ -511,7 +511,7 @@ struct gmapf_s {
         // slot i and admission needs no ranks (the slots are interchangeable:
         // a search's result does not depend on its slot)
         ident = P >= N && N > 0;
-        sp = SearchParams{T, Q, cfg.path_finder_heuristic_k, 0, rpw, ident ? N : 0};
+        sp = SearchParams{T, Q, cfg.path_finder_heuristic_k, 0, rpw, ident ? N : 0, cfg.search_wide_keys ? 1 : 0};
         const int Pe = ident ? N : P;  // slots that can be in flight
         HIPCHK(search_prepare(T, Q));
         if (cfg.step_mode == GMAPF_MODE_BATCHED) {

@@ -160,6 +160,7 @@ struct SearchParams {
     int seg;     // searches per workgroup (segments 0 .. seg-1 take work), 1 .. 64 / T
     int ident;   // > 0: every agent has a slot of its own (slot i = agent i, P >= N);
                  // segment i takes slot i when it is in flight, and ident is the slot count
+    int wide;    // 1: queue keys without G and side, also on maps that fit compact keys
 };
 
 struct CrowdParams {

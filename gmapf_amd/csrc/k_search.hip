@@ -32,11 +32,13 @@
 // state is split by use: a per-slot byte map (m << 1 | dir, m = the search's
 // 7-bit generation, so maps are not cleared per search) answers the probe of
 // every neighbour with one byte, 64 nodes per cache line; an 8-byte entry
-// {G << 1 | dir, parent} is read only for popped nodes and by the corridor
-// walk.  A node's neighbours and their centroids come in one 48-byte record;
-// the record and entry of the node the next round will pop, and the marks of
-// its neighbours, are requested while the current round's frontier is
-// pushed.
+// {G << 1 | dir, parent} is written for every claimed node and read by the
+// corridor walk.  On maps of at most 2^15 nodes the queue keys also carry G
+// and dir (compact keys), so a popped node's entry is not read; wider maps
+// read it with the node's record.  A node's neighbours and their centroids
+// come in one 48-byte record; the record of the node the next round will pop,
+// and the marks of its neighbours, are requested while the current round's
+// frontier is pushed.
 //
 // Forms: plain (one search per segment, run to completion; the agent of a
 // found search moves at once and its corridor is walked beside the other
@@ -72,6 +74,36 @@ __device__ __forceinline__ unsigned long long make_key(float F, int node) {
 }
 
 __device__ __forceinline__ int key_node(unsigned long long k) { return (int)(unsigned)(k & 0xffffffffull); }
+
+// Compact keys (maps of at most 2^15 nodes): the low word holds
+// node << 17 | G << 1 | dir, so the key orders by (F, node) as before (a node
+// enters a search's queues once, so G and dir never decide an order) and a
+// popped key carries the G and direction the round needs: no visited-entry
+// read per expansion.  G < V <= 2^15 fits its 16 bits.
+constexpr int kKeyNodeShift = 17;
+constexpr int kCompactMaxNodes = 1 << 15;
+constexpr unsigned kKeyGdMask = (1u << kKeyNodeShift) - 1u;
+
+template <bool KG>
+__device__ __forceinline__ unsigned long long node_key(float F, int node, int G, int dir) {
+    if constexpr (KG) {
+        return make_key(F, (int)(((unsigned)node << kKeyNodeShift) | ((unsigned)G << 1) | (unsigned)dir));
+    } else {
+        return make_key(F, node);
+    }
+}
+
+template <bool KG>
+__device__ __forceinline__ int key_node_of(unsigned long long k) {
+    if constexpr (KG) {
+        return (int)((unsigned)k >> kKeyNodeShift);
+    } else {
+        return key_node(k);
+    }
+}
+
+// G << 1 | dir of a compact key (the layout of a visited entry's first word)
+__device__ __forceinline__ unsigned key_gd(unsigned long long k) { return (unsigned)k & kKeyGdMask; }
 
 __device__ __forceinline__ float dist2(float2 a, float2 b) {
     float x = a.x - b.x;
@@ -414,8 +446,9 @@ constexpr int search_waves_per_eu() {
 }
 
 // QC > 0 is the queue capacity fixed at compile time (the heap loops unroll
-// with constant bounds); QC = 0 takes it from the launch.
-template <int T, int QC>
+// with constant bounds); QC = 0 takes it from the launch.  KG: compact keys
+// (the map has at most 2^15 nodes; only with QC > 0).
+template <int T, int QC, bool KG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves_per_eu<T>()))) void k_search(DevMap map, DevAgents ag, DevSlots slots, DevFrame fr,
                                                SearchParams sp, int persistent) {
     extern __shared__ unsigned long long q[];  // 64 * (Q + 4) keys: Q heap rows + the spare rows
@@ -461,6 +494,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     // and the items dealt to the lane at the end of the previous round, or
     // popped at the top of the first round of a fresh search
     int c_next = -1;
+    unsigned c_gd = 0u;  // compact keys: G << 1 | dir of c_next
     int S = 0, Tg = 0, cnt = 0;
     unsigned mval = 0;  // this search's mark generation, 1..127
     float2 cS = make_float2(0.f, 0.f), cT = make_float2(0.f, 0.f);
@@ -572,13 +606,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                         cnt = 0;
                         for (int i = 0; i < Q; ++i) q[i * kWave + L] = kKeyNone;
                         if (l == 0) {
-                            q[L] = make_key(F0, S);
+                            q[L] = node_key<KG>(F0, S, 0, 1);
                             cnt = 1;
                             vis[S] = make_uint2(1u, 0xffffffffu);  // G 0, dir 1
                             mark[S] = (unsigned char)((mval << 1) | 1u);
                         }
                         if (l == 1) {
-                            q[L] = make_key(F0, Tg);
+                            q[L] = node_key<KG>(F0, Tg, 0, 0);
                             cnt = 1;
                             vis[Tg] = make_uint2(0u, 0xffffffffu);  // G 0, dir 0
                             mark[Tg] = (unsigned char)(mval << 1);
@@ -594,7 +628,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                         const unsigned long long* src = slots.qsave + qb * (size_t)Q;
                         const unsigned long long k0 = src[0];
                         fl.set(kPoppedNext, k0 != kKeyNone);
-                        c_next = key_node(k0);
+                        c_next = key_node_of<KG>(k0);
+                        c_gd = key_gd(k0);
                         for (int i0 = 0; i0 < Q - 1; i0 += 8) {
                             unsigned long long x[8];
 #pragma unroll
@@ -621,7 +656,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         if (__ballot(fl.get(kHasReq) && fl.get(kNeedPop)) != 0ull) {
             const bool on = fl.get(kHasReq) && fl.get(kNeedPop) && cnt > 0;
             if (on) {
-                c_next = key_node(q[L]);
+                const unsigned long long k0 = q[L];
+                c_next = key_node_of<KG>(k0);
+                c_gd = key_gd(k0);
                 --cnt;
             }
             const int lrow = on ? cnt : Q + kPadRows;
@@ -642,7 +679,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             r0 = map.rec[3 * (size_t)c];
             r1 = map.rec[3 * (size_t)c + 1];
             r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)c + 2);
-            vc = vis[c];
+            if constexpr (KG) {
+                vc.x = c_gd;
+            } else {
+                vc = vis[c];
+            }
         }
         const int G = (int)(vc.x >> 1);
         const int cdir = (int)(vc.x & 1u);
@@ -770,7 +811,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
 #pragma unroll
             for (int s = 0; s < 3; ++s) {
                 if (claim[s]) {
-                    stage[segbase + f] = make_key(Fn[s], nn[s]);
+                    stage[segbase + f] = node_key<KG>(Fn[s], nn[s], G + 1, cdir);
                     mark[nn[s]] = (unsigned char)((mval << 1) | (unsigned)cdir);
                     vis[nn[s]] = make_uint2(((unsigned)(G + 1) << 1) | (unsigned)cdir, (unsigned)c);
                     ++f;
@@ -825,7 +866,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         m = d1 < m ? d1 : m;
         m = d2 < m ? d2 : m;
         const bool has_next = m != kKeyNone;
-        pf_c = has_next ? key_node(m) : -1;
+        pf_c = has_next ? key_node_of<KG>(m) : -1;
         // issued by every lane (node 0 of slot 0 when there is no next node:
         // one shared cache line), so the loads and the wait for them below
         // lie on every path and the compiler can count the loads in flight
@@ -835,7 +876,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         pf_r0 = map.rec[3 * (size_t)pfa];
         pf_r1 = map.rec[3 * (size_t)pfa + 1];
         pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)pfa + 2);
-        pf_vc = pvis[pfa];
+        if constexpr (KG) {
+            pf_vc.x = key_gd(m);
+        } else {
+            pf_vc = pvis[pfa];
+        }
         {
             const bool from_top = has_next && m == top;
             // the root's replacement: the first dealt item, or the last item
@@ -980,21 +1025,24 @@ constexpr bool search_q_fixed(int Q) {
 }
 
 template <int T>
-const void* search_fn_q(int Q) {
+const void* search_fn_q(int Q, bool kg) {
     if constexpr (T <= 8) {
-        if (search_q_fixed<T>(Q)) return Q == 16 ? (const void*)k_search<T, 16> : (const void*)k_search<T, 32>;
+        if (search_q_fixed<T>(Q)) {
+            if (kg) return Q == 16 ? (const void*)k_search<T, 16, true> : (const void*)k_search<T, 32, true>;
+            return Q == 16 ? (const void*)k_search<T, 16, false> : (const void*)k_search<T, 32, false>;
+        }
     }
-    return (const void*)k_search<T, 0>;
+    return (const void*)k_search<T, 0, false>;
 }
 
-const void* search_fn(int T, int Q) {
+const void* search_fn(int T, int Q, bool kg) {
     switch (T) {
-        case 2: return search_fn_q<2>(Q);
-        case 4: return search_fn_q<4>(Q);
-        case 8: return search_fn_q<8>(Q);
-        case 16: return search_fn_q<16>(Q);
-        case 32: return search_fn_q<32>(Q);
-        case 64: return search_fn_q<64>(Q);
+        case 2: return search_fn_q<2>(Q, kg);
+        case 4: return search_fn_q<4>(Q, kg);
+        case 8: return search_fn_q<8>(Q, kg);
+        case 16: return search_fn_q<16>(Q, kg);
+        case 32: return search_fn_q<32>(Q, kg);
+        case 64: return search_fn_q<64>(Q, kg);
         default: return nullptr;
     }
 }
@@ -1004,29 +1052,39 @@ const void* search_fn(int T, int Q) {
 int search_lds_bytes(int Q) { return kWave * (Q + kSpareRows) * (int)sizeof(unsigned long long); }
 
 hipError_t search_prepare(int T, int Q) {
-    const void* fn = search_fn(T, Q);
-    if (!fn) return hipErrorInvalidValue;
-    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, search_lds_bytes(Q));
+    for (const bool kg : {false, true}) {
+        const void* fn = search_fn(T, Q, kg);
+        if (!fn) return hipErrorInvalidValue;
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, search_lds_bytes(Q));
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_search(const DevMap& map, const DevAgents& ag, const DevSlots& slots, const DevFrame& fr,
                          const SearchParams& sp, int grid, int persistent, hipStream_t st) {
     const size_t lds = (size_t)search_lds_bytes(sp.Q);
-#define GMAPF_SEARCH_LAUNCH(TT, QQ)                                                                  \
-    hipLaunchKernelGGL((k_search<TT, QQ>), dim3(grid), dim3(kWave), lds, st, map, ag, slots, fr, sp, \
+    // compact keys when every node id fits their 15-bit field
+    const bool kg = map.V <= kCompactMaxNodes && sp.wide == 0;
+#define GMAPF_SEARCH_LAUNCH(TT, QQ, KK)                                                                  \
+    hipLaunchKernelGGL((k_search<TT, QQ, KK>), dim3(grid), dim3(kWave), lds, st, map, ag, slots, fr, sp, \
                        persistent)
 #define GMAPF_SEARCH_CASE(TT)                          \
     case TT:                                           \
         if (!search_q_fixed<TT>(sp.Q))                 \
-            GMAPF_SEARCH_LAUNCH(TT, 0);                \
+            GMAPF_SEARCH_LAUNCH(TT, 0, false);         \
+        else if (sp.Q == 16 && kg)                     \
+            GMAPF_SEARCH_LAUNCH(TT, 16, true);         \
         else if (sp.Q == 16)                           \
-            GMAPF_SEARCH_LAUNCH(TT, 16);               \
+            GMAPF_SEARCH_LAUNCH(TT, 16, false);        \
+        else if (kg)                                   \
+            GMAPF_SEARCH_LAUNCH(TT, 32, true);         \
         else                                           \
-            GMAPF_SEARCH_LAUNCH(TT, 32);               \
+            GMAPF_SEARCH_LAUNCH(TT, 32, false);        \
         return hipGetLastError();
 #define GMAPF_SEARCH_CASE_GENERIC(TT)                  \
     case TT:                                           \
-        GMAPF_SEARCH_LAUNCH(TT, 0);                    \
+        GMAPF_SEARCH_LAUNCH(TT, 0, false);             \
         return hipGetLastError();
     switch (sp.T) {
         GMAPF_SEARCH_CASE(2)
@@ -1046,7 +1104,7 @@ hipError_t launch_search(const DevMap& map, const DevAgents& ag, const DevSlots&
 hipError_t search_occupancy(int T, int Q, int* blocks_per_cu) {
     hipError_t e = search_prepare(T, Q);
     if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, search_fn(T, Q), kWave,
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, search_fn(T, Q, false), kWave,
                                                         (size_t)search_lds_bytes(Q));
 }
 

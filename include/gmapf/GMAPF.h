@@ -54,6 +54,7 @@ struct Config {
     int SearchesPerWave = 0;     // searches per 64-lane search workgroup; 0: 64 / T
     int SearchTiming = 0;        // time the search launch of every k-th frame; 0: none
     int SearchWorkgroups = 0;    // batched form: cap on the persistent search grid; 0: automatic
+    bool SearchWideKeys = false;  // queue keys without G and side also on maps of <= 32,768 nodes
 };
 
 class CuNodesMap {
@@ -101,6 +102,7 @@ class GMAPF {
         cc.searches_per_wave = c.SearchesPerWave;
         cc.search_timing = c.SearchTiming;
         cc.search_workgroups = c.SearchWorkgroups;
+        cc.search_wide_keys = c.SearchWideKeys ? 1 : 0;
         check(gmapf_init(h_, &cc));
     }
     void ManualStart() { check(gmapf_manual_start(h_)); }

@@ -73,6 +73,11 @@ typedef struct gmapf_config {
                                           searches_per_wave searches in flight, up to what
                                           the GPU holds at once.  A smaller grid makes the
                                           workgroups take searches by atomic ticket */
+    int search_wide_keys;              /* 1: queue keys hold (F, node) only and a popped node's G
+                                          and side are read from its visited entry; 0: on maps
+                                          of at most 32,768 nodes the keys also carry them (no
+                                          visited-entry read per expansion).  Results are the
+                                          same either way */
 } gmapf_config;
 
 /* Reference CuNodesMap::Node (include/gmapf/misc/CuNodesMap.h:14-21);

@@ -12,22 +12,31 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("T,Q,spw", [(2, 16, 0), (4, 16, 0), (8, 32, 0), (16, 8, 0), (32, 4, 0), (64, 2, 0), (4, 1, 0),
-                                     (4, 16, 8), (2, 16, 7), (8, 16, 1),
-                                     # the other compile-time queue kernels and the
-                                     # deeper heaps of the autotuner's grid
-                                     (2, 32, 0), (4, 32, 0), (4, 64, 0), (16, 128, 0), (64, 256, 0)])
-def test_single_frame_search_parity(oracle, T, Q, spw):
+@pytest.mark.parametrize("T,Q,spw,wide", [(2, 16, 0, False), (4, 16, 0, False), (8, 32, 0, False), (16, 8, 0, False),
+                                          (32, 4, 0, False), (64, 2, 0, False), (4, 1, 0, False),
+                                          (4, 16, 8, False), (2, 16, 7, False), (8, 16, 1, False),
+                                          # the other compile-time queue kernels and the
+                                          # deeper heaps of the autotuner's grid
+                                          (2, 32, 0, False), (4, 32, 0, False), (4, 64, 0, False),
+                                          (16, 128, 0, False), (64, 256, 0, False),
+                                          # keys without G and side (the form maps of more
+                                          # than 32,768 nodes take) in the compile-time
+                                          # queue kernels
+                                          (2, 16, 0, True), (4, 16, 0, True), (8, 16, 0, True), (2, 32, 0, True),
+                                          (4, 32, 0, True), (8, 32, 0, True)])
+def test_single_frame_search_parity(oracle, T, Q, spw, wide):
     """One frame, every agent searches once; corridors and flags vs
-    oracle.search.  spw > 0: fewer searches per workgroup than 64 / T."""
+    oracle.search.  spw > 0: fewer searches per workgroup than 64 / T.
+    wide: queue keys without G and side, which then come from the popped
+    node's visited entry."""
     tri, nbr = H.grid_map(11, 48, 40, cell=4.0, p=0.3)
     cen = S.centroids(tri)
     pairs = H.connected_pairs(5, nbr, 300)
     cfg = G.Config(agents_max_count=len(pairs), path_finder_parallel_agents=len(pairs),
                    path_finder_threads_per_agents=T, path_finder_queue_capacity=Q,
-                   agent_radius=0.25, agent_speed=1.0, searches_per_wave=spw)
+                   agent_radius=0.25, agent_speed=1.0, searches_per_wave=spw, search_wide_keys=wide)
     g = H.make_gpu(tri, nbr, cfg, [cen[a] for a, _ in pairs], [cen[b] for _, b in pairs])
-    print(f"search parity T={T} Q={Q} spw={spw} V={len(tri)}", flush=True)
+    print(f"search parity T={T} Q={Q} spw={spw} wide={wide} V={len(tri)}", flush=True)
     g.step(1.0 / 60)
     om = oracle.OracleMap(tri, nbr)
     _, st = g.agents()
@@ -107,7 +116,7 @@ def test_world_parity_on_one_way_neighbour_lists(oracle, hub, mode, budget):
     assert gs["found"] > 50
 
 
-def _world(seed, nx, ny, n_agents, T, Q, mode=G.MODE_PLAIN, budget=0, p=0.2, P=None, spw=0):
+def _world(seed, nx, ny, n_agents, T, Q, mode=G.MODE_PLAIN, budget=0, p=0.2, P=None, spw=0, wide=False):
     tri, nbr = H.grid_map(seed, nx, ny, cell=4.0, p=p)
     cen = S.centroids(tri)
     lab = S.components(nbr)
@@ -120,22 +129,27 @@ def _world(seed, nx, ny, n_agents, T, Q, mode=G.MODE_PLAIN, budget=0, p=0.2, P=N
     cfg = G.Config(agents_max_count=n_agents, path_finder_parallel_agents=P or n_agents,
                    path_finder_threads_per_agents=T, path_finder_queue_capacity=Q,
                    agent_radius=0.5, agent_speed=2.0, step_mode=mode, round_budget=budget or 1,
-                   searches_per_wave=spw)
+                   searches_per_wave=spw, search_wide_keys=wide)
     return tri, nbr, cfg, starts, targs
 
 
-@pytest.mark.parametrize("mode,budget,P,spw,T,Q", [(G.MODE_PLAIN, 0, None, 0, 8, 16), (G.MODE_PLAIN, 0, 37, 0, 8, 16),
-                                                   (G.MODE_BATCHED, 3, None, 0, 8, 16),
-                                                   (G.MODE_BATCHED, 1, 50, 0, 8, 16),
-                                                   (G.MODE_BATCHED, 3, None, 3, 8, 16),
-                                                   # queues parked and restored past 16 items
-                                                   (G.MODE_BATCHED, 2, None, 0, 4, 64),
-                                                   (G.MODE_BATCHED, 3, 60, 0, 2, 32)])
-def test_multi_frame_world_parity(oracle, mode, budget, P, spw, T, Q):
+@pytest.mark.parametrize("mode,budget,P,spw,T,Q,wide", [(G.MODE_PLAIN, 0, None, 0, 8, 16, False),
+                                                        (G.MODE_PLAIN, 0, 37, 0, 8, 16, False),
+                                                        (G.MODE_BATCHED, 3, None, 0, 8, 16, False),
+                                                        (G.MODE_BATCHED, 1, 50, 0, 8, 16, False),
+                                                        (G.MODE_BATCHED, 3, None, 3, 8, 16, False),
+                                                        # queues parked and restored past 16 items
+                                                        (G.MODE_BATCHED, 2, None, 0, 4, 64, False),
+                                                        (G.MODE_BATCHED, 3, 60, 0, 2, 32, False),
+                                                        # keys without G and side, parked and restored
+                                                        (G.MODE_BATCHED, 3, None, 0, 4, 16, True),
+                                                        (G.MODE_BATCHED, 2, 60, 0, 2, 32, True),
+                                                        (G.MODE_PLAIN, 0, 37, 0, 4, 16, True)])
+def test_multi_frame_world_parity(oracle, mode, budget, P, spw, T, Q, wide):
     """Agents search, move, avoid and progress for many frames; every frame
     must equal the oracle bit for bit (positions, states, nodes, corridors)."""
-    tri, nbr, cfg, starts, targs = _world(21, 40, 40, 160, T, Q, mode, budget, P=P, spw=spw)
-    print(f"world parity mode={mode} budget={budget} P={P} spw={spw} T={T} Q={Q}", flush=True)
+    tri, nbr, cfg, starts, targs = _world(21, 40, 40, 160, T, Q, mode, budget, P=P, spw=spw, wide=wide)
+    print(f"world parity mode={mode} budget={budget} P={P} spw={spw} T={T} Q={Q} wide={wide}", flush=True)
     g = H.make_gpu(tri, nbr, cfg, starts, targs)
     w = H.make_oracle(oracle, tri, nbr, cfg, starts, targs)
     dt = 1.0 / 60
