@@ -584,17 +584,36 @@ void triangulate_map(const std::vector<float>& vxy, const std::vector<int>& edge
             }
         }
     }
-    // keep odd depth, renumber in creation order
-    std::vector<int> newid(nt, -1);
-    int nk = 0;
+    // keep odd depth; number the kept triangles along a Hilbert curve over
+    // their centroids (creation order breaks ties), so that a triangle's
+    // neighbours get nearby ids: a search probes the per-node visited bytes of
+    // a node's neighbours, and nearby ids put them on the node's own cache line
+    std::vector<std::pair<uint64_t, int>> kept;
+    double cx0 = 0.0, cy0 = 0.0, cx1 = 1.0, cy1 = 1.0;
+    for (int i = 0; i < nv; i++) {
+        const P2& p = cdt.pts[i];
+        if (i == 0 || p.x < cx0) cx0 = p.x;
+        if (i == 0 || p.y < cy0) cy0 = p.y;
+        if (i == 0 || p.x > cx1) cx1 = p.x;
+        if (i == 0 || p.y > cy1) cy1 = p.y;
+    }
+    const double span = std::max(cx1 - cx0, cy1 - cy0) > 0.0 ? std::max(cx1 - cx0, cy1 - cy0) : 1.0;
     for (int t = 0; t < nt; t++) {
         if (!cdt.tris[t].alive || depth[t] == INT32_MAX || depth[t] % 2 == 0) continue;
         const Tri& T = cdt.tris[t];
         bool super = false;
         for (int k = 0; k < 3; k++) super = super || (T.v[k] >= nv && T.v[k] < nv + 3);
         if (super) continue;
-        newid[t] = nk++;
+        const double mx = (cdt.pts[T.v[0]].x + cdt.pts[T.v[1]].x + cdt.pts[T.v[2]].x) / 3.0;
+        const double my = (cdt.pts[T.v[0]].y + cdt.pts[T.v[1]].y + cdt.pts[T.v[2]].y) / 3.0;
+        const double fx = std::min(std::max((mx - cx0) / span, 0.0), 1.0);
+        const double fy = std::min(std::max((my - cy0) / span, 0.0), 1.0);
+        kept.push_back({hilbert((uint32_t)(fx * 65535.0), (uint32_t)(fy * 65535.0), 16), t});
     }
+    std::sort(kept.begin(), kept.end());
+    std::vector<int> newid(nt, -1);
+    int nk = 0;
+    for (const auto& kt : kept) newid[kt.second] = nk++;
     if (nk == 0) throw std::runtime_error("no walkable triangles");
     tri6.assign(6 * (size_t)nk, 0.f);
     nbr3.assign(3 * (size_t)nk, -1);
