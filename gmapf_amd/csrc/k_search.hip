@@ -426,10 +426,9 @@ enum : unsigned {
     kHasReq = 1u,       // the segment holds a search or a corridor walk
     kNoMore = 2u,       // the segment has no more work to take
     kTookPlain = 4u,    // the segment took its one item (one search per segment)
-    kNeedPop = 8u,      // the lane pops its first node at the top of the round
-    kPoppedNext = 16u,  // the lane has a node to expand in the coming round
-    kWalking = 32u,     // batched form: the segment walks a found search's corridor
-    kPfVnOk = 64u,      // the prefetched marks belong to the coming node
+    kPoppedNext = 8u,   // the lane has a node to expand in the coming round (its record
+                        // and the marks of its neighbours are requested)
+    kWalking = 16u,     // batched form: the segment walks a found search's corridor
 };
 struct LaneFlags {
     unsigned v = 0u;
@@ -556,10 +555,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                     vis = slots.vis + (size_t)slot * V;
                     mark = slots.mark + (size_t)slot * slots.VP;
                     pf_c = -1;
-                    fl.set(kPfVnOk, false);
-                    fl.set(kNeedPop, status == 3);
                     fl.set(kPoppedNext, false);
                     fl.set(kHasReq, true);
+                    const bool seeded = status == 3;
                     if (status == 4) {
                         // found earlier, corridor deferred for lack of slab space
                         const bool hold = sp.budget > 0;
@@ -640,6 +638,38 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                         }
                         q[(Q - 1) * kWave + L] = kKeyNone;
                     }
+                    // The node the first round expands: a seed is its heap's
+                    // only item, so popping it empties the heap; a parked
+                    // queue keeps it in front.  Its record and the marks of
+                    // its neighbours are requested here, so every round finds
+                    // them prefetched (the barrier orders the seeds' marks
+                    // before the loads of the other lanes of the segment).
+                    if (seeded) {
+                        fl.set(kPoppedNext, cnt > 0);
+                        if (cnt > 0) {
+                            const unsigned long long k0 = q[L];
+                            c_next = key_node_of<KG>(k0);
+                            c_gd = key_gd(k0);
+                            q[L] = kKeyNone;
+                            cnt = 0;
+                        }
+                    }
+                    wave_sync();
+                    if (fl.get(kPoppedNext)) {
+                        pf_c = c_next;
+                        pf_r0 = map.rec[3 * (size_t)c_next];
+                        pf_r1 = map.rec[3 * (size_t)c_next + 1];
+                        pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)c_next + 2);
+                        if constexpr (KG) {
+                            pf_vc.x = c_gd;
+                        } else {
+                            pf_vc = vis[c_next];
+                        }
+                        const int a0 = pf_r0.x, a1 = a0 >= 0 ? pf_r0.y : -1, a2 = a1 >= 0 ? pf_r0.z : -1;
+                        pf_vn[0] = mark[a0 >= 0 ? a0 : 0];
+                        pf_vn[1] = mark[a1 >= 0 ? a1 : 0];
+                        pf_vn[2] = mark[a2 >= 0 ? a2 : 0];
+                    }
                 }
             }
         }
@@ -652,39 +682,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         if (__ballot(fl.get(kHasReq) || !fl.get(kNoMore)) == 0ull) break;
         wave_sync();  // seeds, restored queues and visited writes visible
 
-        // ---- pop (first round of a fresh search) + neighbour fetch
-        if (__ballot(fl.get(kHasReq) && fl.get(kNeedPop)) != 0ull) {
-            const bool on = fl.get(kHasReq) && fl.get(kNeedPop) && cnt > 0;
-            if (on) {
-                const unsigned long long k0 = q[L];
-                c_next = key_node_of<KG>(k0);
-                c_gd = key_gd(k0);
-                --cnt;
-            }
-            const int lrow = on ? cnt : Q + kPadRows;
-            const unsigned long long last = q[lrow * kWave + L];
-            q[lrow * kWave + L] = kKeyNone;  // the vacated entry
-            heap_sift_down(q, L, Q, depth, on && cnt > 0, last);
-            if (fl.get(kHasReq) && fl.get(kNeedPop)) fl.set(kPoppedNext, on);
-            fl.set(kNeedPop, false);
-        }
+        // ---- the node this lane expands: its record, its G and side and the
+        // marks of its neighbours were requested at the end of the last round
+        // (or when the search was taken)
         const bool popped = fl.get(kHasReq) && fl.get(kPoppedNext);
         const int c = popped ? c_next : -1;
-        const bool miss = popped && c != pf_c;  // first round of a search: nothing prefetched
-        const bool have_vn = popped && fl.get(kPfVnOk) && !miss;
-        int4 r0 = pf_r0, r1 = pf_r1;
-        int2 r2 = pf_r2;
-        uint2 vc = pf_vc;
-        if (__ballot(miss) != 0ull && miss) {
-            r0 = map.rec[3 * (size_t)c];
-            r1 = map.rec[3 * (size_t)c + 1];
-            r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)c + 2);
-            if constexpr (KG) {
-                vc.x = c_gd;
-            } else {
-                vc = vis[c];
-            }
-        }
+        const int4 r0 = pf_r0, r1 = pf_r1;
+        const int2 r2 = pf_r2;
+        const uint2 vc = pf_vc;
         const int G = (int)(vc.x >> 1);
         const int cdir = (int)(vc.x & 1u);
         int nn[3];
@@ -694,13 +699,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         const float2 cnb[3] = {make_float2(__int_as_float(r1.x), __int_as_float(r1.y)),
                                make_float2(__int_as_float(r1.z), __int_as_float(r1.w)),
                                make_float2(__int_as_float(r2.x), __int_as_float(r2.y))};
-        // marks of the neighbours: prefetched, or read now (all three loads
-        // issued before the first is used)
-        unsigned mk[3] = {pf_vn[0], pf_vn[1], pf_vn[2]};
-        if (__ballot(popped && !have_vn) != 0ull && popped && !have_vn) {
-#pragma unroll
-            for (int s = 0; s < 3; ++s) mk[s] = mark[nn[s] >= 0 ? nn[s] : 0];
-        }
+        const unsigned mk[3] = {pf_vn[0], pf_vn[1], pf_vn[2]};  // marks of the neighbours
         // F of every slot (a missing neighbour's value is not used); h is
         // measured to the opposite endpoint of the popped node's direction
         const float2 cfar = cdir ? cT : cS;
@@ -909,7 +908,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                 pf_vn[0] = pmark[a0 >= 0 ? a0 : 0];
                 pf_vn[1] = pmark[a1 >= 0 ? a1 : 0];
                 pf_vn[2] = pmark[a2 >= 0 ? a2 : 0];
-                fl.set(kPfVnOk, has_next);
             }
             // the items still to push, at most two: the dealt items other
             // than m (and d0 when it replaced the root)
