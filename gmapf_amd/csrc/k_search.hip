@@ -498,8 +498,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     unsigned mval = 0;  // this search's mark generation, 1..127
     float2 cS = make_float2(0.f, 0.f), cT = make_float2(0.f, 0.f);
     // rounds_here counts the segment's rounds in this launch (search and
-    // corridor walk, for the budget), srch_here its search rounds
-    int rounds_here = 0, srch_here = 0, exp_here = 0, disc_here = 0;
+    // corridor walk, for the budget)
+    int rounds_here = 0;
     // batched form: the segment walks its search's corridor (cbase, clen)
     int cbase = 0, clen = 0;
     int status = 0;
@@ -516,171 +516,172 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     // frame statistics, added to fr.stats once per workgroup at the end
     unsigned st_found = 0, st_fail = 0, st_exp = 0, st_disc = 0, st_corr = 0, st_rounds = 0;
 
+    // Acquisition and the test for the end of the workgroup run only when a
+    // segment may need work: in the first round, after a segment's search or
+    // walk ended or parked, and while a ticket gave a segment nothing.
+    bool acquire = true;
     while (true) {
-        // ---- acquire work for segments that have none (one wave-uniform
-        // test when every segment is busy; rare paths of the round are
-        // skipped the same way, so a round pays no lane-mask bookkeeping for
-        // them)
-        if (__ballot(!fl.get(kHasReq) && !fl.get(kNoMore)) != 0ull) {
-            if (!fl.get(kHasReq) && !fl.get(kNoMore)) {
-                int item = -1;
-                if (persistent && !one_each) {
-                    if (l == 0) item = atomicAdd(&fr.counters[2], 1);
-                    item = __shfl(item, sub * T, kWave);
-                } else if (!fl.get(kTookPlain)) {
-                    // every search has a segment of its own: segment i takes
-                    // item i, once (the plain form always does)
-                    item = blockIdx.x * sp.seg + sub;
-                    fl.set(kTookPlain, true);
-                }
-                // slot = agent: nothing to do when no search is in flight there
-                if (sp.ident > 0 && item >= 0 && item < n_active && slots.agent[item] < 0) item = -1;
-                if (item < 0 || item >= n_active) {
-                    fl.set(kNoMore, true);
-                } else {
-                    slot = sp.ident > 0 ? item : fr.active[(int)(((long long)item * stride) % n_active)];
-                    agent = slots.agent[slot];
-                    S = slots.S[slot];
-                    Tg = slots.Tg[slot];
-                    const unsigned stamp = slots.stamp[slot];
-                    mval = stamp % 127u + 1u;
-                    status = slots.status[slot];
-                    cS = map.cen[S];
-                    cT = map.cen[Tg];
-                    rounds_here = 0;
-                    srch_here = 0;
-                    fl.set(kWalking, false);
-                    exp_here = 0;
-                    disc_here = 0;
-                    vis = slots.vis + (size_t)slot * V;
-                    mark = slots.mark + (size_t)slot * slots.VP;
-                    pf_c = -1;
-                    fl.set(kPoppedNext, false);
-                    fl.set(kHasReq, true);
-                    const bool seeded = status == 3;
-                    if (status == 4) {
-                        // found earlier, corridor deferred for lack of slab space
-                        const bool hold = sp.budget > 0;
-                        const bool ok = start_corridor(l, slot, agent, S, Tg, slots.mS[slot], slots.mT[slot], vis, slab,
-                                                       ag, slots, V, &wshare[3 * sub], walk, hold, cbase, clen, st_found,
-                                                       st_corr);
-                        fl.set(kWalking, hold && ok);
-                        fl.set(kHasReq, fl.get(kWalking));
-                        status = 0;
-                    } else if (status == 5) {
-                        // batched form: a corridor walk parked at the end of the
-                        // last launch resumes
-                        const int4 pw = slots.walk[slot];
-                        const int2 cr = slots.corr[slot];
-                        cbase = cr.x;
-                        clen = cr.y;
-                        if (l < 2) {
-                            const int n = l == 0 ? pw.x : pw.z;
-                            walk.active = n >= 0;
-                            walk.vis = vis;
-                            walk.n = n;
-                            walk.term = l == 0 ? S : Tg;
-                            walk.idx = cbase + (l == 0 ? pw.y : pw.w);
-                            if (walk.active) walk.ent = vis[n];
-                        }
-                        fl.set(kWalking, true);
-                        status = 0;
-                    } else if (status == 3) {
-                        // Stamps 127e .. 127e + 126 (epoch e) use generations
-                        // 1 .. 127 once each.  The first search of an epoch to
-                        // seed in this slot zeroes the slot's marks before its
-                        // seeds, so only marks of the current epoch remain; a
-                        // search admitted and dropped before it seeded (its
-                        // agent re-targeted) cannot skip the clear.
-                        const unsigned epoch = stamp / 127u;
-                        if (slots.clr[slot] != epoch) {
-                            uint4* m4 = reinterpret_cast<uint4*>(mark);
-                            for (long long w = l; w < slots.VP / 16; w += T) m4[w] = make_uint4(0u, 0u, 0u, 0u);
-                            wave_sync();
-                            if (l == 0) slots.clr[slot] = epoch;
-                        }
-                        // PathFinder.h:168-189
-                        const float F0 = dist2(cS, cT) * sp.K;
-                        cnt = 0;
-                        for (int i = 0; i < Q; ++i) q[i * kWave + L] = kKeyNone;
-                        if (l == 0) {
-                            q[L] = node_key<KG>(F0, S, 0, 1);
-                            cnt = 1;
-                            vis[S] = make_uint2(1u, 0xffffffffu);  // G 0, dir 1
-                            mark[S] = (unsigned char)((mval << 1) | 1u);
-                        }
-                        if (l == 1) {
-                            q[L] = node_key<KG>(F0, Tg, 0, 0);
-                            cnt = 1;
-                            vis[Tg] = make_uint2(0u, 0xffffffffu);  // G 0, dir 0
-                            mark[Tg] = (unsigned char)(mval << 1);
-                        }
-                        status = 0;
-                    } else {
-                        // parked: [node to expand next or kKeyNone, heap items]
-                        // (at most Q - 1 heap items).  Loads go in groups of 8
-                        // with clamped in-bounds indices and the count applied
-                        // afterwards, so a group waits for memory once
-                        const size_t qb = ((size_t)slot * T + l);
-                        cnt = slots.qcnt[qb];
-                        const unsigned long long* src = slots.qsave + qb * (size_t)Q;
-                        const unsigned long long k0 = src[0];
-                        fl.set(kPoppedNext, k0 != kKeyNone);
-                        c_next = key_node_of<KG>(k0);
-                        c_gd = key_gd(k0);
-                        for (int i0 = 0; i0 < Q - 1; i0 += 8) {
-                            unsigned long long x[8];
-#pragma unroll
-                            for (int j = 0; j < 8; ++j) x[j] = src[1 + min(i0 + j, Q - 2)];
-#pragma unroll
-                            for (int j = 0; j < 8; ++j)
-                                if (i0 + j < Q - 1) q[(i0 + j) * kWave + L] = i0 + j < cnt ? x[j] : kKeyNone;
-                        }
-                        q[(Q - 1) * kWave + L] = kKeyNone;
+        if (acquire) {
+            // ---- acquire work for segments that have none
+            if (__ballot(!fl.get(kHasReq) && !fl.get(kNoMore)) != 0ull) {
+                if (!fl.get(kHasReq) && !fl.get(kNoMore)) {
+                    int item = -1;
+                    if (persistent && !one_each) {
+                        if (l == 0) item = atomicAdd(&fr.counters[2], 1);
+                        item = __shfl(item, sub * T, kWave);
+                    } else if (!fl.get(kTookPlain)) {
+                        // every search has a segment of its own: segment i takes
+                        // item i, once (the plain form always does)
+                        item = blockIdx.x * sp.seg + sub;
+                        fl.set(kTookPlain, true);
                     }
-                    // The node the first round expands: a seed is its heap's
-                    // only item, so popping it empties the heap; a parked
-                    // queue keeps it in front.  Its record and the marks of
-                    // its neighbours are requested here, so every round finds
-                    // them prefetched (the barrier orders the seeds' marks
-                    // before the loads of the other lanes of the segment).
-                    if (seeded) {
-                        fl.set(kPoppedNext, cnt > 0);
-                        if (cnt > 0) {
-                            const unsigned long long k0 = q[L];
+                    // slot = agent: nothing to do when no search is in flight there
+                    if (sp.ident > 0 && item >= 0 && item < n_active && slots.agent[item] < 0) item = -1;
+                    if (item < 0 || item >= n_active) {
+                        fl.set(kNoMore, true);
+                    } else {
+                        slot = sp.ident > 0 ? item : fr.active[(int)(((long long)item * stride) % n_active)];
+                        agent = slots.agent[slot];
+                        S = slots.S[slot];
+                        Tg = slots.Tg[slot];
+                        const unsigned stamp = slots.stamp[slot];
+                        mval = stamp % 127u + 1u;
+                        status = slots.status[slot];
+                        cS = map.cen[S];
+                        cT = map.cen[Tg];
+                        rounds_here = 0;
+                        fl.set(kWalking, false);
+                        vis = slots.vis + (size_t)slot * V;
+                        mark = slots.mark + (size_t)slot * slots.VP;
+                        pf_c = -1;
+                        fl.set(kPoppedNext, false);
+                        fl.set(kHasReq, true);
+                        const bool seeded = status == 3;
+                        if (status == 4) {
+                            // found earlier, corridor deferred for lack of slab space
+                            const bool hold = sp.budget > 0;
+                            const bool ok =
+                                start_corridor(l, slot, agent, S, Tg, slots.mS[slot], slots.mT[slot], vis, slab, ag,
+                                               slots, V, &wshare[3 * sub], walk, hold, cbase, clen, st_found, st_corr);
+                            fl.set(kWalking, hold && ok);
+                            fl.set(kHasReq, fl.get(kWalking));
+                            status = 0;
+                        } else if (status == 5) {
+                            // batched form: a corridor walk parked at the end of the
+                            // last launch resumes
+                            const int4 pw = slots.walk[slot];
+                            const int2 cr = slots.corr[slot];
+                            cbase = cr.x;
+                            clen = cr.y;
+                            if (l < 2) {
+                                const int n = l == 0 ? pw.x : pw.z;
+                                walk.active = n >= 0;
+                                walk.vis = vis;
+                                walk.n = n;
+                                walk.term = l == 0 ? S : Tg;
+                                walk.idx = cbase + (l == 0 ? pw.y : pw.w);
+                                if (walk.active) walk.ent = vis[n];
+                            }
+                            fl.set(kWalking, true);
+                            status = 0;
+                        } else if (status == 3) {
+                            // Stamps 127e .. 127e + 126 (epoch e) use generations
+                            // 1 .. 127 once each.  The first search of an epoch to
+                            // seed in this slot zeroes the slot's marks before its
+                            // seeds, so only marks of the current epoch remain; a
+                            // search admitted and dropped before it seeded (its
+                            // agent re-targeted) cannot skip the clear.
+                            const unsigned epoch = stamp / 127u;
+                            if (slots.clr[slot] != epoch) {
+                                uint4* m4 = reinterpret_cast<uint4*>(mark);
+                                for (long long w = l; w < slots.VP / 16; w += T) m4[w] = make_uint4(0u, 0u, 0u, 0u);
+                                wave_sync();
+                                if (l == 0) slots.clr[slot] = epoch;
+                            }
+                            // PathFinder.h:168-189
+                            const float F0 = dist2(cS, cT) * sp.K;
+                            cnt = 0;
+                            for (int i = 0; i < Q; ++i) q[i * kWave + L] = kKeyNone;
+                            if (l == 0) {
+                                q[L] = node_key<KG>(F0, S, 0, 1);
+                                cnt = 1;
+                                vis[S] = make_uint2(1u, 0xffffffffu);  // G 0, dir 1
+                                mark[S] = (unsigned char)((mval << 1) | 1u);
+                            }
+                            if (l == 1) {
+                                q[L] = node_key<KG>(F0, Tg, 0, 0);
+                                cnt = 1;
+                                vis[Tg] = make_uint2(0u, 0xffffffffu);  // G 0, dir 0
+                                mark[Tg] = (unsigned char)(mval << 1);
+                            }
+                            status = 0;
+                        } else {
+                            // parked: [node to expand next or kKeyNone, heap items]
+                            // (at most Q - 1 heap items).  Loads go in groups of 8
+                            // with clamped in-bounds indices and the count applied
+                            // afterwards, so a group waits for memory once
+                            const size_t qb = ((size_t)slot * T + l);
+                            cnt = slots.qcnt[qb];
+                            const unsigned long long* src = slots.qsave + qb * (size_t)Q;
+                            const unsigned long long k0 = src[0];
+                            fl.set(kPoppedNext, k0 != kKeyNone);
                             c_next = key_node_of<KG>(k0);
                             c_gd = key_gd(k0);
-                            q[L] = kKeyNone;
-                            cnt = 0;
+                            for (int i0 = 0; i0 < Q - 1; i0 += 8) {
+                                unsigned long long x[8];
+    #pragma unroll
+                                for (int j = 0; j < 8; ++j) x[j] = src[1 + min(i0 + j, Q - 2)];
+    #pragma unroll
+                                for (int j = 0; j < 8; ++j)
+                                    if (i0 + j < Q - 1) q[(i0 + j) * kWave + L] = i0 + j < cnt ? x[j] : kKeyNone;
+                            }
+                            q[(Q - 1) * kWave + L] = kKeyNone;
                         }
-                    }
-                    wave_sync();
-                    if (fl.get(kPoppedNext)) {
-                        pf_c = c_next;
-                        pf_r0 = map.rec[3 * (size_t)c_next];
-                        pf_r1 = map.rec[3 * (size_t)c_next + 1];
-                        pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)c_next + 2);
-                        if constexpr (KG) {
-                            pf_vc.x = c_gd;
-                        } else {
-                            pf_vc = vis[c_next];
+                        // The node the first round expands: a seed is its heap's
+                        // only item, so popping it empties the heap; a parked
+                        // queue keeps it in front.  Its record and the marks of
+                        // its neighbours are requested here, so every round finds
+                        // them prefetched (the barrier orders the seeds' marks
+                        // before the loads of the other lanes of the segment).
+                        if (seeded) {
+                            fl.set(kPoppedNext, cnt > 0);
+                            if (cnt > 0) {
+                                const unsigned long long k0 = q[L];
+                                c_next = key_node_of<KG>(k0);
+                                c_gd = key_gd(k0);
+                                q[L] = kKeyNone;
+                                cnt = 0;
+                            }
                         }
-                        const int a0 = pf_r0.x, a1 = a0 >= 0 ? pf_r0.y : -1, a2 = a1 >= 0 ? pf_r0.z : -1;
-                        pf_vn[0] = mark[a0 >= 0 ? a0 : 0];
-                        pf_vn[1] = mark[a1 >= 0 ? a1 : 0];
-                        pf_vn[2] = mark[a2 >= 0 ? a2 : 0];
+                        wave_sync();
+                        if (fl.get(kPoppedNext)) {
+                            pf_c = c_next;
+                            pf_r0 = map.rec[3 * (size_t)c_next];
+                            pf_r1 = map.rec[3 * (size_t)c_next + 1];
+                            pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)c_next + 2);
+                            if constexpr (KG) {
+                                pf_vc.x = c_gd;
+                            } else {
+                                pf_vc = vis[c_next];
+                            }
+                            const int a0 = pf_r0.x, a1 = a0 >= 0 ? pf_r0.y : -1, a2 = a1 >= 0 ? pf_r0.z : -1;
+                            pf_vn[0] = mark[a0 >= 0 ? a0 : 0];
+                            pf_vn[1] = mark[a1 >= 0 ? a1 : 0];
+                            pf_vn[2] = mark[a2 >= 0 ? a2 : 0];
+                        }
                     }
                 }
             }
+            // A segment whose ticket gave it nothing to run (a deferred corridor
+            // that still does not fit) passes this round idle and takes the next
+            // ticket, so the workgroup leaves only when no segment has work and
+            // every segment's tickets are spent.  (A loop around the acquisition
+            // would do the same without the idle round, but its code made every
+            // round 9% slower.)
+            if (__ballot(fl.get(kHasReq) || !fl.get(kNoMore)) == 0ull) break;
+            acquire = __ballot(!fl.get(kHasReq) && !fl.get(kNoMore)) != 0ull;
+            wave_sync();  // seeds, restored queues and visited writes visible
         }
-        // A segment whose ticket gave it nothing to run (a deferred corridor
-        // that still does not fit) passes this round idle and takes the next
-        // ticket, so the workgroup leaves only when no segment has work and
-        // every segment's tickets are spent.  (A loop around the acquisition
-        // would do the same without the idle round, but its code made every
-        // round 9% slower.)
-        if (__ballot(fl.get(kHasReq) || !fl.get(kNoMore)) == 0ull) break;
-        wave_sync();  // seeds, restored queues and visited writes visible
 
         // ---- the node this lane expands: its record, its G and side and the
         // marks of its neighbours were requested at the end of the last round
@@ -831,7 +832,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             mS = __shfl(a, w, kWave);
             mT = __shfl(b, w, kWave);
         }
-        const unsigned long long pb = __ballot(popped) & smask;
         wave_sync();
         if constexpr (T > 4) {
             // empty this round's table slots for the next round
@@ -922,14 +922,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         const unsigned long long nb = __ballot(has_next) & smask;
         const bool searching = fl.get(kHasReq) && !fl.get(kWalking);
         rounds_here += fl.get(kHasReq) ? 1 : 0;
-        srch_here += searching ? 1 : 0;
-        exp_here += searching ? __popcll(pb) : 0;
-        disc_here += searching ? tot : 0;
+        // frame statistics per lane (summed over the wave at the end): the
+        // lane's expansion and claims, and the segment's round on lane 0
+        st_exp += searching && popped ? 1u : 0u;
+        st_disc += searching ? (unsigned)cl : 0u;
+        st_rounds += searching && l == 0 ? 1u : 0u;
         status = !searching ? status : (found ? 1 : (nb == 0ull ? 2 : status));
         // batched form: a walking segment is done when both of its sides are
         const bool wdone = fl.get(kWalking) && (__ballot(walk.active) & smask) == 0ull;
         const bool ended = (searching && status != 0) || wdone;
         const bool park = fl.get(kHasReq) && !ended && sp.budget > 0 && rounds_here >= sp.budget;
+        if (__ballot(ended || park) != 0ull) acquire = true;
         if (__ballot(ended || park) != 0ull && (ended || park)) {
             if (park && !fl.get(kWalking)) {
                 // [node to expand next or kKeyNone, heap items]: at most Q keys
@@ -943,9 +946,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             if (l == 0) {
                 if (searching) {
                     slots.status[slot] = status;
-                    st_exp += (unsigned)exp_here;
-                    st_disc += (unsigned)disc_here;
-                    st_rounds += (unsigned)srch_here;
                     if (ended && status == 2) {  // PathFinder.h:268-274
                         ag.state[agent] = kIdle;
                         ++st_fail;
