@@ -240,28 +240,38 @@ __device__ __forceinline__ unsigned claim_hash(int n, int bits) {
 
 // First candidate of each of a lane's three neighbour slots among the
 // candidates of its segment that come earlier in (lane, slot) order.  A
-// candidate is (node << 1 | direction) when `ins` (the node exists and this
-// search has not marked it), else absent.  first[s] is the earliest earlier
-// candidate for the same node, or -1 when slot s's candidate is the node's
-// first.  Lanes l - 1 .. l - (T - 1) are read with DPP row shifts (a segment
-// of T <= 16 lanes lies inside one 16-lane row; the lanes of the previous
-// segment that a shift also reaches are masked by l >= n), nearest lane
-// first so that the earliest match is the one kept.
+// candidate that is present (`ins`: the node exists and this search has not
+// marked it) is keyed k = (node << SB | r) + 1, r the segment's index inside
+// its 16-lane DPP row (SB bits), and packed as k << 1 | direction; an absent
+// one has key -1 and packed value 0.  first[s] is the packed value of the
+// earliest earlier candidate for the same node, or -1 when slot s's candidate
+// is the node's first.  Lanes l - 1 .. l - (T - 1) are read with DPP row
+// shifts (a segment of T <= 16 lanes lies inside one 16-lane row), nearest
+// lane first so that the earliest match is the one kept.  A shift that
+// reaches the previous segment brings keys with another r, and one that
+// leaves the row brings 0 (bound_ctrl), so neither can match: no mask of
+// the lanes below n is needed.
+template <int T>
+constexpr int seg_row_bits() {
+    int b = 0;
+    while ((T << b) < 16) ++b;
+    return b;
+}
+
 template <int N>
 __device__ __forceinline__ int row_shr(int v) {
-    // row_shr:N; a lane with no source lane in its row gets -2 (absent)
-    return __builtin_amdgcn_update_dpp(-2, v, 0x110 + N, 0xf, 0xf, false);
+    // row_shr:N; a lane with no source lane in its row gets 0 (absent)
+    return __builtin_amdgcn_mov_dpp(v, 0x110 + N, 0xf, 0xf, true);
 }
 
 template <int T, int N>
-__device__ __forceinline__ void seg_first_from(int l, const int* key, const int* pk, int* first) {
+__device__ __forceinline__ void seg_first_from(const int* key, const int* pk, int* first) {
     if constexpr (N < T) {
-        const bool v = l >= N;
         int wp[3], ws[3];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
             wp[s] = row_shr<N>(pk[s]);
-            ws[s] = v ? (wp[s] >> 1) : -1;  // absent: -1, never equal to a key
+            ws[s] = wp[s] >> 1;  // 0 when absent: never equal to a key
         }
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
@@ -269,25 +279,26 @@ __device__ __forceinline__ void seg_first_from(int l, const int* key, const int*
             first[s] = key[s] == ws[1] ? wp[1] : first[s];
             first[s] = key[s] == ws[0] ? wp[0] : first[s];
         }
-        seg_first_from<T, N + 1>(l, key, pk, first);
+        seg_first_from<T, N + 1>(key, pk, first);
     }
 }
 
 template <int T>
-__device__ __forceinline__ void seg_first_candidates(int l, const int* nn, const bool* ins, int cdir, int* first) {
-    int pk[3], key[3], own[3];
+__device__ __forceinline__ void seg_first_candidates(int sub, const int* nn, const bool* ins, int cdir, int* first) {
+    constexpr int SB = seg_row_bits<T>();
+    const int r = sub & ((1 << SB) - 1);
+    int pk[3], key[3];
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
-        pk[s] = ins[s] ? ((nn[s] << 1) | cdir) : -2;
-        own[s] = ins[s] ? nn[s] : -1;
-        key[s] = ins[s] ? nn[s] : -3;  // matches nothing when absent
+        key[s] = ins[s] ? ((nn[s] << SB) | r) + 1 : -1;
+        pk[s] = ins[s] ? ((key[s] << 1) | cdir) : 0;
     }
     // earlier slots of the lane itself
     first[0] = -1;
-    first[1] = key[1] == own[0] ? pk[0] : -1;
-    first[2] = key[2] == own[1] ? pk[1] : -1;
-    first[2] = key[2] == own[0] ? pk[0] : first[2];
-    seg_first_from<T, 1>(l, key, pk, first);
+    first[1] = key[1] == (pk[0] >> 1) ? pk[0] : -1;
+    first[2] = key[2] == (pk[1] >> 1) ? pk[1] : -1;
+    first[2] = key[2] == (pk[0] >> 1) ? pk[0] : first[2];
+    seg_first_from<T, 1>(key, pk, first);
 }
 
 // Corridor walk of one side of a found search (PathFinder.h:276-306), one
@@ -744,7 +755,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
             // row shift by n brings lane l - n's candidates to lane l (at
             // T = 8 the 63 compares per lane cost more than the table)
             int first[3];
-            seg_first_candidates<T>(l, nn, ins, cdir, first);
+            seg_first_candidates<T>(sub, nn, ins, cdir, first);
 #pragma unroll
             for (int s = 0; s < 3; ++s) {
                 claim[s] = ins[s] && first[s] < 0;

@@ -25,6 +25,9 @@ inline int cell_of(float v, float o, float inv, int n) {
 
 void Navmesh::assign(int v, const float* tri6, const int* nbr3) {
     if (v <= 0) throw std::runtime_error("navmesh has no nodes");
+    // the search packs a node id with a few bits of segment and direction
+    // into one 32-bit word (and every search slot holds 9 bytes per node)
+    if (v > kMaxNodes) throw std::runtime_error("navmesh has more than 2^26 nodes");
     V = v;
     tri.assign(tri6, tri6 + 6 * (size_t)v);
     nbr.assign(3 * (size_t)v, -1);

@@ -6,6 +6,9 @@
 
 namespace gmapf_amd {
 
+// Largest navmesh the step takes (FillMap* and the GMAPFNAV loader reject more).
+constexpr int kMaxNodes = 1 << 26;
+
 struct Navmesh {
     int V = 0;
     std::vector<float> tri;  // 6V: p1x p1y p2x p2y p3x p3y
@@ -23,6 +26,7 @@ struct Navmesh {
     // Build from triangles + neighbours; centroids ((P1+P2)+P3)/3.0f
     // (GMAPF.cu:212-216).  Neighbour lists are kept as given (-1 = none).
     // Throws std::runtime_error on malformed input.
+    // at most kMaxNodes nodes
     void assign(int v, const float* tri6, const int* nbr3);
     void build_grid();
     // CuNodesMap.h:216-229: minimum index of a triangle containing (x, y), -1 if none.

@@ -102,7 +102,7 @@ int gmapf_manual_start(gmapf_handle h);                           /* GMAPF::Manu
 int gmapf_fill_map_start(gmapf_handle h, float width, float height);   /* GMAPF.cu:133 */
 int gmapf_fill_map_obstacle(gmapf_handle h, const float* xy, int n);   /* GMAPF.cu:147, n vertices */
 int gmapf_fill_map_end(gmapf_handle h);                                /* GMAPF.cu:161 */
-int gmapf_fill_map_manually(gmapf_handle h, const gmapf_node* nodes, int n); /* GMAPF.cu:199 */
+int gmapf_fill_map_manually(gmapf_handle h, const gmapf_node* nodes, int n); /* GMAPF.cu:199; n <= 2^26 */
 
 /* ---- map fixture (no reference counterpart): a filled navmesh as one
  * versioned, checksummed byte string ("GMAPFNAV" v1: vertices, centroids,
