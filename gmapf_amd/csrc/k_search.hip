@@ -301,6 +301,29 @@ __device__ __forceinline__ void seg_first_candidates(int sub, const int* nn, con
     seg_first_from<T, 1>(key, pk, first);
 }
 
+// Exclusive prefix `pre` of the claims `cl` over the lanes of the segment
+// (lane order) and the segment's total `tot`.  T = 2 and 4: the segment is a
+// DPP pair or quad; an inclusive scan of shifted adds, each shift fed only by
+// lanes whose target lies in the same segment (the others contribute 0),
+// then the segment's last lane broadcast with a quad permute.  Wider
+// segments: two ballots of the claim count's bits and popcounts.
+template <int T>
+__device__ __forceinline__ void seg_scan(int l, int cl, unsigned long long below, unsigned long long smask, int& pre,
+                                         int& tot) {
+    if constexpr (T == 2 || T == 4) {
+        int inc = cl + __builtin_amdgcn_mov_dpp(l == T - 1 ? 0 : cl, 0x111, 0xf, 0xf, true);  // row_shr:1
+        if constexpr (T == 4) inc += __builtin_amdgcn_mov_dpp(l >= 2 ? 0 : inc, 0x112, 0xf, 0xf, true);  // row_shr:2
+        pre = inc - cl;
+        // quad_perm [1,1,3,3] (pairs) or [3,3,3,3] (quads): the segment's last lane
+        tot = __builtin_amdgcn_mov_dpp(inc, T == 4 ? 0xff : 0xf5, 0xf, 0xf, false);
+    } else {
+        const unsigned long long b0 = __ballot(cl & 1);
+        const unsigned long long b1 = __ballot(cl & 2);
+        pre = __popcll(b0 & below) + 2 * __popcll(b1 & below);
+        tot = __popcll(b0 & smask) + 2 * __popcll(b1 & smask);
+    }
+}
+
 // Corridor walk of one side of a found search (PathFinder.h:276-306), one
 // parent step per call so the walk of a search that ended overlaps the rounds
 // of the other searches of the workgroup.  The corridor occupies
@@ -813,10 +836,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
 
         // ---- claims: frontier order = (lane, slot) order
         const int cl = (int)claim[0] + (int)claim[1] + (int)claim[2];
-        const unsigned long long b0 = __ballot(cl & 1);
-        const unsigned long long b1 = __ballot(cl & 2);
-        const int pre = __popcll(b0 & below) + 2 * __popcll(b1 & below);
-        const int tot = __popcll(b0 & smask) + 2 * __popcll(b1 & smask);
+        int pre, tot;
+        seg_scan<T>(l, cl, below, smask, pre, tot);
         {
             int f = pre;
 #pragma unroll
