@@ -231,8 +231,10 @@ struct gmapf_s {
         if (Q < 1 || Q > 256) throw ApiError(GMAPF_E_ARG, "PathFinderQueueCapacity must be in [1, 256]");
         if (cfg.path_finder_parallel_agents < 1)
             throw ApiError(GMAPF_E_ARG, "PathFinderParallelAgents must be >= 1");
-        if (!std::isfinite(cfg.path_finder_heuristic_k))
-            throw ApiError(GMAPF_E_ARG, "PathFinderHeuristicK must be finite");
+        // a negative weight would rank nodes farther from the other end first;
+        // with K >= 0 every F is at least 1, which the search's keys rely on
+        if (!std::isfinite(cfg.path_finder_heuristic_k) || cfg.path_finder_heuristic_k < 0.0f)
+            throw ApiError(GMAPF_E_ARG, "PathFinderHeuristicK must be finite and >= 0");
         if (cfg.step_mode != GMAPF_MODE_PLAIN && cfg.step_mode != GMAPF_MODE_BATCHED)
             throw ApiError(GMAPF_E_ARG, "unknown step_mode");
         if (cfg.step_mode == GMAPF_MODE_BATCHED && cfg.round_budget < 1)

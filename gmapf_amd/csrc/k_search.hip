@@ -66,11 +66,14 @@ constexpr int kWave = 64;
 // lgkmcnt in 11:8)
 constexpr int kWaitVmem0 = 0x0f70;
 
-__device__ __forceinline__ unsigned long long make_key(float F, int node) {
-    F = F + 0.0f;  // -0 -> +0 so equal floats give equal keys
-    unsigned b = __float_as_uint(F);
-    b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-    return ((unsigned long long)b << 32) | (unsigned)node;
+// Queue key of an item: F's bits with the sign bit set (the order of
+// non-negative floats as unsigned integers, above every bit pattern with the
+// sign clear), then the low word.  ManualStart refuses a negative or
+// non-finite HeuristicK, so F = (G + 1) + h * K >= 1 for every pushed item and
+// the seeds' F = h * K >= +0; the general map of a signed float (all bits
+// inverted when negative, -0 folded to +0) would give the same keys.
+__device__ __forceinline__ unsigned long long make_key(float F, unsigned low) {
+    return ((unsigned long long)(__float_as_uint(F) | 0x80000000u) << 32) | low;
 }
 
 __device__ __forceinline__ int key_node(unsigned long long k) { return (int)(unsigned)(k & 0xffffffffull); }
@@ -87,9 +90,9 @@ constexpr unsigned kKeyGdMask = (1u << kKeyNodeShift) - 1u;
 template <bool KG>
 __device__ __forceinline__ unsigned long long node_key(float F, int node, int G, int dir) {
     if constexpr (KG) {
-        return make_key(F, (int)(((unsigned)node << kKeyNodeShift) | ((unsigned)G << 1) | (unsigned)dir));
+        return make_key(F, ((unsigned)node << kKeyNodeShift) | ((unsigned)G << 1) | (unsigned)dir);
     } else {
-        return make_key(F, node);
+        return make_key(F, (unsigned)node);
     }
 }
 

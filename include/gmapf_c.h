@@ -44,7 +44,7 @@ typedef struct gmapf_config {
     int path_finder_parallel_agents;   /* Config::PathFinderParallelAgents (P) */
     int path_finder_threads_per_agents;/* Config::PathFinderThreadsPerAgents (T: 2,4,..,64) */
     int path_finder_queue_capacity;    /* Config::PathFinderQueueCapacity (Q) */
-    float path_finder_heuristic_k;     /* Config::PathFinderHeuristicK (K) */
+    float path_finder_heuristic_k;     /* Config::PathFinderHeuristicK (K); finite and >= 0 */
     int is_debug;                      /* Config::IsDebug: sync + check after every phase */
     int is_profiler;                   /* Config::IsProfiler: per-phase event timing */
     /* build knobs, no reference counterpart */

@@ -94,6 +94,21 @@ def test_searches_per_wave_bounded_by_lanes():
     g.close()
 
 
+@pytest.mark.parametrize("k", [-1.0, -1e-30, float("nan"), float("inf")])
+def test_heuristic_weight_must_be_finite_and_not_negative(k):
+    """HeuristicK < 0 or not finite is refused at ManualStart (the
+    reference takes any float; a negative weight ranks nodes farther from
+    the other end first, and the search's queue keys assume F >= 1)."""
+    g = G.GMAPF()
+    g.init(G.Config(agents_max_count=1, path_finder_heuristic_k=k))
+    tri, nbr = H.grid_map(1, 6, 6, cell=2.0, p=0.0)
+    g.fill_map_manually(tri, nbr)
+    with pytest.raises(G.GmapfError) as e:
+        g.manual_start()
+    assert e.value.code == -1
+    g.close()
+
+
 def test_negative_search_timing_rejected():
     g = G.GMAPF()
     g.init(G.Config(agents_max_count=1, search_timing=-1))

@@ -52,6 +52,32 @@ def test_single_frame_search_parity(oracle, T, Q, spw, wide):
     assert nfound > 0
 
 
+@pytest.mark.parametrize("K", [0.0, 0.25, 3.0])
+def test_search_parity_at_other_heuristic_weights(oracle, K):
+    """HeuristicK 0 (the seeds' F is +0, every other F is G + 1), below and
+    above 1: flags and corridors equal the oracle's."""
+    tri, nbr = H.grid_map(13, 40, 40, cell=4.0, p=0.3)
+    cen = S.centroids(tri)
+    pairs = H.connected_pairs(7, nbr, 200)
+    cfg = G.Config(agents_max_count=len(pairs), path_finder_parallel_agents=len(pairs),
+                   path_finder_threads_per_agents=4, path_finder_queue_capacity=16, path_finder_heuristic_k=K,
+                   agent_radius=0.25, agent_speed=1.0)
+    g = H.make_gpu(tri, nbr, cfg, [cen[a] for a, _ in pairs], [cen[b] for _, b in pairs])
+    print(f"search parity K={K} V={len(tri)}", flush=True)
+    g.step(1.0 / 60)
+    om = oracle.OracleMap(tri, nbr)
+    _, st = g.agents()
+    nfound = 0
+    for i, (a, b) in enumerate(pairs):
+        found, cor, _ = om.search(a, b, 4, 16, K)
+        assert st[i] == (G.MOVE if found else G.IDLE), f"agent {i} ({a}->{b}) state"
+        if found:
+            nfound += 1
+            assert np.array_equal(g.corridor(i), cor), f"agent {i} ({a}->{b}) corridor"
+    assert nfound > 0
+    g.close()
+
+
 def _one_way_map(hub):
     """A grid navmesh whose neighbour lists are no longer a mesh's (the
     FillMapManually input may be any lists): the back link of about one edge
