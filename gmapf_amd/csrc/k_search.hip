@@ -572,28 +572,64 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                         item = blockIdx.x * sp.seg + sub;
                         fl.set(kTookPlain, true);
                     }
+                    // The item's slot.  Its fields, and the count and front of
+                    // a queue parked there, are requested together; then the
+                    // centroids and the first node's record; then (after the
+                    // seeds) its neighbours' marks: three waits for memory.
+                    int slot_i = -1;
+                    if (item >= 0 && item < n_active)
+                        slot_i = sp.ident > 0 ? item : fr.active[(int)(((long long)item * stride) % n_active)];
+                    const size_t qb = (size_t)(slot_i >= 0 ? slot_i : 0) * T + l;
+                    int agent_i = -1, S_i = 0, Tg_i = 0, status_i = 0, qcnt_i = 0;
+                    unsigned stamp = 0u;
+                    unsigned long long k0_i = kKeyNone;
+                    if (slot_i >= 0) {
+                        agent_i = slots.agent[slot_i];
+                        S_i = slots.S[slot_i];
+                        Tg_i = slots.Tg[slot_i];
+                        stamp = slots.stamp[slot_i];
+                        status_i = slots.status[slot_i];
+                        qcnt_i = slots.qcnt[qb];
+                        k0_i = slots.qsave[qb * (size_t)Q];
+                    }
                     // slot = agent: nothing to do when no search is in flight there
-                    if (sp.ident > 0 && item >= 0 && item < n_active && slots.agent[item] < 0) item = -1;
-                    if (item < 0 || item >= n_active) {
+                    if (sp.ident > 0 && agent_i < 0) slot_i = -1;
+                    if (slot_i < 0) {
                         fl.set(kNoMore, true);
                     } else {
-                        slot = sp.ident > 0 ? item : fr.active[(int)(((long long)item * stride) % n_active)];
-                        agent = slots.agent[slot];
-                        S = slots.S[slot];
-                        Tg = slots.Tg[slot];
-                        const unsigned stamp = slots.stamp[slot];
+                        slot = slot_i;
+                        agent = agent_i;
+                        S = S_i;
+                        Tg = Tg_i;
                         mval = stamp % 127u + 1u;
-                        status = slots.status[slot];
+                        status = status_i;
                         cS = map.cen[S];
                         cT = map.cen[Tg];
                         rounds_here = 0;
                         fl.set(kWalking, false);
                         vis = slots.vis + (size_t)slot * V;
                         mark = slots.mark + (size_t)slot * slots.VP;
-                        pf_c = -1;
-                        fl.set(kPoppedNext, false);
                         fl.set(kHasReq, true);
                         const bool seeded = status == 3;
+                        // The node the first round expands: a new search's
+                        // seeds (start on lane 0 with G 0 and side 1, target on
+                        // lane 1 with side 0, each its heap's only item and so
+                        // popped at once: PathFinder.h:168-189) or a parked
+                        // queue's front.  Its record is requested now.
+                        fl.set(kPoppedNext, seeded ? l < 2 : status == 0 && k0_i != kKeyNone);
+                        c_next = seeded ? (l == 0 ? S : Tg) : key_node_of<KG>(k0_i);
+                        c_gd = seeded ? (l == 0 ? 1u : 0u) : key_gd(k0_i);
+                        pf_c = fl.get(kPoppedNext) ? c_next : -1;
+                        if (fl.get(kPoppedNext)) {
+                            pf_r0 = map.rec[3 * (size_t)c_next];
+                            pf_r1 = map.rec[3 * (size_t)c_next + 1];
+                            pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)c_next + 2);
+                            if constexpr (KG) {
+                                pf_vc.x = c_gd;
+                            } else {
+                                pf_vc = seeded ? make_uint2(c_gd, 0xffffffffu) : vis[c_next];
+                            }
+                        }
                         if (status == 4) {
                             // found earlier, corridor deferred for lack of slab space
                             const bool hold = sp.budget > 0;
@@ -635,72 +671,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                                 wave_sync();
                                 if (l == 0) slots.clr[slot] = epoch;
                             }
-                            // PathFinder.h:168-189
-                            const float F0 = dist2(cS, cT) * sp.K;
+                            // the seeds (their queue keys are not needed: each is
+                            // popped at once, above, and its heap stays empty)
                             cnt = 0;
                             for (int i = 0; i < Q; ++i) q[i * kWave + L] = kKeyNone;
                             if (l == 0) {
-                                q[L] = node_key<KG>(F0, S, 0, 1);
-                                cnt = 1;
                                 vis[S] = make_uint2(1u, 0xffffffffu);  // G 0, dir 1
                                 mark[S] = (unsigned char)((mval << 1) | 1u);
                             }
                             if (l == 1) {
-                                q[L] = node_key<KG>(F0, Tg, 0, 0);
-                                cnt = 1;
                                 vis[Tg] = make_uint2(0u, 0xffffffffu);  // G 0, dir 0
                                 mark[Tg] = (unsigned char)(mval << 1);
                             }
                             status = 0;
                         } else {
                             // parked: [node to expand next or kKeyNone, heap items]
-                            // (at most Q - 1 heap items).  Loads go in groups of 8
-                            // with clamped in-bounds indices and the count applied
+                            // (at most Q - 1 heap items; the front and the count
+                            // were read above).  Loads go in groups of 8 with
+                            // clamped in-bounds indices and the count applied
                             // afterwards, so a group waits for memory once
-                            const size_t qb = ((size_t)slot * T + l);
-                            cnt = slots.qcnt[qb];
+                            cnt = qcnt_i;
                             const unsigned long long* src = slots.qsave + qb * (size_t)Q;
-                            const unsigned long long k0 = src[0];
-                            fl.set(kPoppedNext, k0 != kKeyNone);
-                            c_next = key_node_of<KG>(k0);
-                            c_gd = key_gd(k0);
                             for (int i0 = 0; i0 < Q - 1; i0 += 8) {
                                 unsigned long long x[8];
-    #pragma unroll
+#pragma unroll
                                 for (int j = 0; j < 8; ++j) x[j] = src[1 + min(i0 + j, Q - 2)];
-    #pragma unroll
+#pragma unroll
                                 for (int j = 0; j < 8; ++j)
                                     if (i0 + j < Q - 1) q[(i0 + j) * kWave + L] = i0 + j < cnt ? x[j] : kKeyNone;
                             }
                             q[(Q - 1) * kWave + L] = kKeyNone;
                         }
-                        // The node the first round expands: a seed is its heap's
-                        // only item, so popping it empties the heap; a parked
-                        // queue keeps it in front.  Its record and the marks of
-                        // its neighbours are requested here, so every round finds
-                        // them prefetched (the barrier orders the seeds' marks
-                        // before the loads of the other lanes of the segment).
-                        if (seeded) {
-                            fl.set(kPoppedNext, cnt > 0);
-                            if (cnt > 0) {
-                                const unsigned long long k0 = q[L];
-                                c_next = key_node_of<KG>(k0);
-                                c_gd = key_gd(k0);
-                                q[L] = kKeyNone;
-                                cnt = 0;
-                            }
-                        }
+                        // the marks of the first node's neighbours, so that every
+                        // round finds them prefetched (the barrier orders the
+                        // seeds' marks before the loads of the segment's other
+                        // lanes)
                         wave_sync();
                         if (fl.get(kPoppedNext)) {
-                            pf_c = c_next;
-                            pf_r0 = map.rec[3 * (size_t)c_next];
-                            pf_r1 = map.rec[3 * (size_t)c_next + 1];
-                            pf_r2 = *reinterpret_cast<const int2*>(map.rec + 3 * (size_t)c_next + 2);
-                            if constexpr (KG) {
-                                pf_vc.x = c_gd;
-                            } else {
-                                pf_vc = vis[c_next];
-                            }
                             const int a0 = pf_r0.x, a1 = a0 >= 0 ? pf_r0.y : -1, a2 = a1 >= 0 ? pf_r0.z : -1;
                             pf_vn[0] = mark[a0 >= 0 ? a0 : 0];
                             pf_vn[1] = mark[a1 >= 0 ? a1 : 0];
