@@ -273,17 +273,23 @@ def default_hash_cell(radius):
 
 
 def crowd_frames(sc, cell, warmup, steps, budget):
-    """Frame times (ms) of the crowd workload with the given hash cell."""
+    """Frame times (ms) of the crowd workload with the given hash cell, the
+    mean host split of a timed frame (enqueue, wait for the device, readback)
+    and the frame kinds (all frames, frames without admission and search)."""
     g = make_world(sc, G.MODE_BATCHED, sc.T, sc.Q, budget, hash_cell=cell)
-    fm = []
+    fm, split = [], []
     for f in range(warmup + steps):
         g.async_step(sc.dt)
         g.wait_step_end()
+        lm = g.last_frame_ms()
         if f >= warmup:
-            fm.append(g.last_frame_ms()[0])
+            fm.append(lm[0])
+            split.append(lm[5:8])
         if f % 100 == 0:
-            log(f"crowd cell {cell:.4g} frame {f}: {g.last_frame_ms()[0]:.3f} ms")
+            log(f"crowd cell {cell:.4g} frame {f}: {lm[0]:.3f} ms")
     st_final = g.stats()
+    st_final["host_split_ms"] = [round(float(v), 4) for v in np.mean(split, axis=0)]
+    st_final["frame_kinds"] = g.frame_kinds()
     g.close()
     return np.array(fm), st_final
 
@@ -341,6 +347,8 @@ def crowd_main(args):
             "crowd_phase_ms": round(tot_ms / 30, 4), "crowd_bytes_per_frame": round(tot_b / 30, 1),
             "neighbour_candidates_per_frame": int(np.mean(cands)),
             "searches_found": st_final["found"], "T": T, "Q": Q, "round_budget": B,
+            "host_split_ms": dict(zip(("enqueue", "device_wait", "readback"), st_final["host_split_ms"])),
+            "frames_without_search": st_final["frame_kinds"]["crowd_only"],
             "default_cell": {"hash_cell": round(dcell, 4), "stencil": "3x3",
                              "agent_steps_per_s": round(sc.n * len(fd) / (fd.sum() / 1e3), 1),
                              "p50_ms": round(float(np.percentile(fd, 50)), 4),

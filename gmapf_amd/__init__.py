@@ -148,6 +148,7 @@ _SIGS = {
     "gmapf_last_frame_ms": [C.c_void_p, C.c_void_p],
     "gmapf_get_hash": [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)],
     "gmapf_get_launch_info": [C.c_void_p, C.c_void_p],
+    "gmapf_get_frame_kinds": [C.c_void_p, C.c_void_p],
     "gmapf_get_slab_info": [C.c_void_p, C.c_void_p],
     "gmapf_get_config": [C.c_void_p, C.POINTER(_Config)],
     "gmapf_agent_states": [C.c_void_p, C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)],
@@ -418,6 +419,13 @@ class GMAPF:
         out = (C.c_longlong * 4)()
         self._check(self._lib.gmapf_get_slab_info(self._h, out))
         return dict(capacity=out[0], in_use=out[1], compactions=out[2], deferred=out[3])
+
+    def frame_kinds(self):
+        """Frames run so far, and those among them that had nothing to admit
+        or search (the crowd phase and the readback only)."""
+        out = np.zeros(2, dtype=np.int64)
+        self._check(self._lib.gmapf_get_frame_kinds(self._h, _ptr(out)))
+        return dict(frames=int(out[0]), crowd_only=int(out[1]))
 
     def launch_info(self):
         out = np.zeros(8, dtype=np.int32)

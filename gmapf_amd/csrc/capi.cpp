@@ -121,12 +121,19 @@ struct gmapf_s {
     int* h_state_stage = nullptr;
     unsigned long long* h_stats = nullptr;
     unsigned long long* h_slab = nullptr;  // [0] slab ints in use, [1] deferred corridors
+    int* h_search = nullptr;  // agents in Search after the last frame, per 64 agents
+    int searching_after = -1;  // their sum (-1: no frame ran yet)
+    long long crowd_frames = 0;  // frames that ran without admission and search
     int* h_upd = nullptr;       // [0] count, then ids[N], targ[N]
     float2* h_upd_xy = nullptr;
     int upd_cap = 0;
     // batched frame graphs, [compaction before the search][search timed]
     hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    // the frame with nothing to admit or search: one stream, the moving set,
+    // the crowd phase and the export (instantiated on its first use)
+    hipGraph_t graph_crowd = nullptr;
+    hipGraphExec_t gexec_crowd = nullptr;
     bool compact_next = false;   // the slab was half full after the last frame
     long long compactions = 0, slab_in_use = 0, slab_deferred = 0;
 
@@ -180,6 +187,10 @@ struct gmapf_s {
                 gexec[c][t] = nullptr;
                 graph[c][t] = nullptr;
             }
+        if (gexec_crowd) hipGraphExecDestroy(gexec_crowd);
+        if (graph_crowd) hipGraphDestroy(graph_crowd);
+        gexec_crowd = nullptr;
+        graph_crowd = nullptr;
         for (void* p : dallocs) hipFree(p);
         dallocs.clear();
         if (h_step) hipHostFree(h_step);
@@ -187,6 +198,9 @@ struct gmapf_s {
         if (h_state_stage) hipHostFree(h_state_stage);
         if (h_stats) hipHostFree(h_stats);
         if (h_slab) hipHostFree(h_slab);
+        if (h_search) hipHostFree(h_search);
+        h_search = nullptr;
+        searching_after = -1;
         if (h_upd) hipHostFree(h_upd);
         if (h_upd_xy) hipHostFree(h_upd_xy);
         h_step = h_pos_stage = nullptr;
@@ -488,6 +502,9 @@ struct gmapf_s {
         HIPCHK(hipHostMalloc((void**)&h_stats, 8 * sizeof(unsigned long long)));
         HIPCHK(hipHostMalloc((void**)&h_slab, 2 * sizeof(unsigned long long)));
         h_slab[0] = h_slab[1] = 0;
+        HIPCHK(hipHostMalloc((void**)&h_search, (size_t)std::max(1, (N + 63) / 64) * sizeof(int)));
+        std::memset(h_search, 0, (size_t)std::max(1, (N + 63) / 64) * sizeof(int));
+        searching_after = -1;
         upd_cap = std::max(N, 1);
         HIPCHK(hipHostMalloc((void**)&h_upd, (1 + 2 * (size_t)upd_cap) * sizeof(int)));
         HIPCHK(hipHostMalloc((void**)&h_upd_xy, (size_t)upd_cap * sizeof(float2)));
@@ -506,6 +523,7 @@ struct gmapf_s {
         dfr.h_state_out = h_state_stage;
         dfr.h_stats_out = h_stats;
         dfr.h_slab_out = h_slab;
+        dfr.h_search_out = h_search;
 
         // ---- search launch shape
         const int rpw = cfg.searches_per_wave > 0 ? cfg.searches_per_wave : 64 / T;
@@ -577,6 +595,29 @@ struct gmapf_s {
         HIPCHK(launch_export(dag, dfr, s1));
     }
 
+    // A frame after which no agent searched, with no target change since:
+    // admission and the search have nothing to do, so the frame is the moving
+    // set, the crowd phase and the export, on one stream.
+    void enqueue_crowd_frame() {
+        HIPCHK(launch_crowd_start(dag, dfr, s1));
+        HIPCHK(launch_crowd(dmap, dag, dfr, cp, s1));
+        HIPCHK(launch_export(dag, dfr, s1));
+    }
+
+    void build_crowd_graph() {
+        HIPCHK(hipStreamBeginCapture(s1, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_crowd_frame();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            hipStreamEndCapture(s1, &g);
+            if (g) hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(s1, &graph_crowd));
+        HIPCHK(hipGraphInstantiate(&gexec_crowd, graph_crowd, nullptr, nullptr, 0));
+    }
+
     // Captures one frame (with or without the slab compaction in front, with
     // or without events around the search) into a graph and instantiates it.
     void build_graph(bool compact, bool timed) {
@@ -612,15 +653,27 @@ struct gmapf_s {
         // last frame left it half full (the host reads the slab top that the
         // last frame exported; no device round trip)
         const bool compact = compact_next;
+        // nothing to admit or search: no agent searched after the last frame
+        // and no target changed since (a compaction waits for a full frame)
+        const bool crowd_only = searching_after == 0 && upd_n == 0 && !compact && !cfg.is_profiler && !cfg.is_debug;
         if (compact) compactions++;
-        const bool timed = cfg.search_timing > 0 && frames % cfg.search_timing == 0;
+        const bool timed = !crowd_only && cfg.search_timing > 0 && frames % cfg.search_timing == 0;
         step_timed = timed;
-        if (cfg.step_mode == GMAPF_MODE_BATCHED && !cfg.is_profiler && !cfg.is_debug) {
+        if (crowd_only) {
+            crowd_frames++;
+            if (cfg.step_mode == GMAPF_MODE_BATCHED) {
+                if (!gexec_crowd) build_crowd_graph();
+                HIPCHK(hipGraphLaunch(gexec_crowd, s1));
+            } else {
+                enqueue_crowd_frame();
+            }
+        } else if (cfg.step_mode == GMAPF_MODE_BATCHED && !cfg.is_profiler && !cfg.is_debug) {
             // every frame variant in use is captured on the first step, so a
             // later frame never pays for an instantiation
             for (int c = 0; c < 2; c++)
                 for (int t = 0; t < (cfg.search_timing > 0 ? 2 : 1); t++)
                     if (!gexec[c][t]) build_graph(c != 0, t != 0);
+            if (!gexec_crowd) build_crowd_graph();
             HIPCHK(hipGraphLaunch(gexec[compact][timed], s1));
         } else if (cfg.is_profiler || cfg.is_debug) {
             enqueue_frame_profiled(compact);
@@ -652,6 +705,8 @@ struct gmapf_s {
         for (int k = 0; k < 8; k++) stats_host[k] = (long long)h_stats[k];
         slab_in_use = (long long)h_slab[0];
         slab_deferred = (long long)h_slab[1];
+        searching_after = 0;
+        for (int w = 0; w < (n_agents + 63) / 64; w++) searching_after += h_search[w];
         compact_next = 2 * slab_in_use > dag.slab_cap;
         frames++;
         stats_host[7] = frames;
@@ -1261,6 +1316,15 @@ int gmapf_get_slab_info(gmapf_handle h, long long* out4) {
         out4[1] = std::min(h->slab_in_use, h->started ? h->dag.slab_cap : 0LL);
         out4[2] = h->compactions;
         out4[3] = h->slab_deferred;
+    });
+}
+
+int gmapf_get_frame_kinds(gmapf_handle h, long long* out2) {
+    return guarded(h, [&] {
+        h->require_init();
+        if (!out2) throw ApiError(GMAPF_E_ARG, "null output");
+        out2[0] = h->frames;
+        out2[1] = h->crowd_frames;
     });
 }
 

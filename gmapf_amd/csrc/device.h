@@ -118,7 +118,7 @@ struct DevFrame {
     int* adm_agent;  // P: agent of the k-th admission this frame
     int* active;     // P: active slot list for the search kernel
     int* counters;   // [0] n_requests [1] n_active [2] work ticket [3] n_ended
-                     // [4] n_free [5] moving count [6] n_admitted
+                     // [4] n_free [5] moving count (0 between frames) [6] n_admitted
                      // [7] unused
                      // [8] unused
                      // [9] ticket stride: ticket t takes active[(t * stride) % n_active]
@@ -147,6 +147,7 @@ struct DevFrame {
     int* state_prev;             // [N] device copy of h_state_out
     unsigned long long* h_stats_out;  // [8] cumulative counters
     unsigned long long* h_slab_out;   // [2] slab top, deferred corridors
+    int* h_search_out;                // [ceil(N / 64)] agents in Search after the frame, per 64 agents
     // corridor slab compaction scratch
     int* slab_len;                    // N + 1: live corridor lengths, then their offsets
     int* slab_blk;                    // scan partials

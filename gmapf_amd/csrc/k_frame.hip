@@ -436,6 +436,7 @@ __global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {
         *fr.stepv = *fr.h_step;
+        fr.counters[2] = 0;  // search tickets (the ranked admission clears it too)
         fr.counters[5] = 0;  // counted by the admission pass that follows
         fr.counters[6] = 0;
     }
@@ -476,8 +477,32 @@ __global__ void k_export(DevAgents ag, DevFrame fr) {
         fr.h_state_out[i] = s;
         if (changed) fr.state_prev[i] = s;
     }
+    // agents left searching, per 64-agent group (the host adds them up: a
+    // frame after which none searches and before which no target changed
+    // needs no admission or search)
+    const unsigned long long srch = __ballot(i < ag.n && s == kSearch);
+    if ((threadIdx.x & 63) == 0 && i < ag.n) fr.h_search_out[i >> 6] = __popcll(srch);
     if (i < 8) fr.h_stats_out[i] = fr.stats[i];
     if (i < 2) fr.h_slab_out[i] = ag.slab_top[i];  // [0] top, [1] deferred (adjacent words)
+    // the movers of this frame are counted; the next frame counts from 0
+    // (its admission pass, or its crowd-only start, adds them up again)
+    if (i == 0) fr.counters[5] = 0;
+}
+
+// First kernel of a crowd-only frame (no agent searches and no target
+// changed since the last frame, so admission and the search have nothing to
+// do): the frame's step and its moving set, which admission takes otherwise
+// (the movers are counted from the 0 the last frame's export left).
+__global__ void k_crowd_start(DevAgents ag, DevFrame fr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *fr.stepv = *fr.h_step;
+        fr.counters[6] = 0;
+    }
+    const bool mv = i < ag.n && ag.state[i] == kMove;
+    if (i < ag.n) ag.in_m[i] = mv ? 1 : 0;
+    const unsigned long long b = __ballot(mv);
+    if ((threadIdx.x & 63) == 0 && b != 0ull) atomicAdd(&fr.counters[5], (int)__popcll(b));
 }
 
 // ------------------------------------------------------------- corridor slab
@@ -861,6 +886,11 @@ hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& 
 
 hipError_t launch_export(const DevAgents& ag, const DevFrame& fr, hipStream_t st) {
     hipLaunchKernelGGL(k_export, dim3(nblk(ag.n > 8 ? ag.n : 8, 256)), dim3(256), 0, st, ag, fr);
+    return hipGetLastError();
+}
+
+hipError_t launch_crowd_start(const DevAgents& ag, const DevFrame& fr, hipStream_t st) {
+    hipLaunchKernelGGL(k_crowd_start, dim3(nblk(ag.n > 0 ? ag.n : 1, 256)), dim3(256), 0, st, ag, fr);
     return hipGetLastError();
 }
 

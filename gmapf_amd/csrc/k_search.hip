@@ -46,8 +46,9 @@
 // searches from an atomic ticket over the active list, or one each when the
 // grid holds a segment per search, run up to `budget` rounds, park their
 // queues in HBM and take the next ticket).  When every agent has a slot of
-// its own (P >= N) there is no active list: segment i takes slot i if a
-// search is in flight there.  In the persistent form a found
+// its own (P >= N) there is no active list: the items are the slots, and
+// a segment given a slot with no search in flight there has nothing to do
+// (one each) or takes the next ticket.  In the persistent form a found
 // search keeps its segment and walks its corridor inside the same budget,
 // two parent steps per round, parking the walk in its slot when the budget
 // runs out; the agent moves when the corridor is complete.
@@ -517,7 +518,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     for (int r = 0; r < kPadRows; ++r) q[(Q + r) * kWave + threadIdx.x] = kKeyNone;
     const size_t V = (size_t)map.V;
     const int n_active = sp.ident > 0 ? sp.ident : fr.counters[1];
-    const bool one_each = sp.ident > 0 || (long long)n_active <= (long long)gridDim.x * sp.seg;
+    // one segment per search when the grid holds that many, else tickets
+    // (also when every agent has a slot of its own: with more agents than
+    // the grid holds segments, the slots past the grid are reached by ticket)
+    const bool one_each = (long long)n_active <= (long long)gridDim.x * sp.seg;
     const long long stride = fr.counters[9];
     // corridor slab in use (compaction runs between launches)
     global_int* const slab = (global_int*)*ag.slab_now;
@@ -592,10 +596,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
                         qcnt_i = slots.qcnt[qb];
                         k0_i = slots.qsave[qb * (size_t)Q];
                     }
-                    // slot = agent: nothing to do when no search is in flight there
-                    if (sp.ident > 0 && agent_i < 0) slot_i = -1;
+                    // slot = agent: nothing to do when no search is in flight
+                    // there; with tickets the segment takes the next one
+                    const bool idle_slot = sp.ident > 0 && slot_i >= 0 && agent_i < 0;
+                    if (idle_slot) slot_i = -1;
                     if (slot_i < 0) {
-                        fl.set(kNoMore, true);
+                        fl.set(kNoMore, !idle_slot || !persistent || one_each);
                     } else {
                         slot = slot_i;
                         agent = agent_i;

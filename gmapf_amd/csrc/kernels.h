@@ -27,6 +27,8 @@ hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& 
                         bool ident, hipStream_t st);
 // positions, states and counters into pinned host memory
 hipError_t launch_export(const DevAgents& ag, const DevFrame& fr, hipStream_t st);
+// first kernel of a frame in which nothing is admitted or searched
+hipError_t launch_crowd_start(const DevAgents& ag, const DevFrame& fr, hipStream_t st);
 // corridor slab compaction: corridors of moving agents into the other buffer
 // in agent order; the others are released
 hipError_t launch_compact(const DevAgents& ag, const DevSlots& sl, const DevFrame& fr, hipStream_t st);

@@ -55,6 +55,7 @@ def run_pair(oracle, sc, mode=G.MODE_PLAIN, budget=1, frames=None, check_every=1
     for k in ("found", "not_found", "expansions", "discoveries", "corridor_nodes", "rounds", "admitted"):
         assert gs[k] == os_[k], (k, gs[k], os_[k])
     xy, st = g.agents()
+    gs["frame_kinds"] = g.frame_kinds()
     g.close()
     return gs, xy, st
 
@@ -66,6 +67,7 @@ def test_config1_bowtie(oracle, mode, budget):
     print(f"config1 mode={mode}", flush=True)
     gs, xy, st = run_pair(oracle, sc, mode, budget)
     assert gs["found"] > 0
+    assert gs["frame_kinds"]["crowd_only"] > 0  # once every search ended
 
 
 def test_config2_rects_full(oracle):
@@ -98,6 +100,23 @@ def test_config4_full_equals_oracle(oracle):
     gs, xy, st = run_pair(oracle, sc, frames=600, check_every=60, hash_cell=4.0)
     assert gs["found"] > 0.9 * sc.n
     assert not np.isnan(xy).any()
+    # after the first frame's searches nothing is admitted or searched: the
+    # frames ran as the crowd phase alone, and still equal the oracle
+    fk = gs["frame_kinds"]
+    assert fk["frames"] == 600 and fk["crowd_only"] >= 590, fk
+
+
+def test_config4_full_batched_equals_oracle(oracle):
+    """The crowd workload at full size in the batched form (round budget 256,
+    as its benchmark runs it): 65,536 agents with a slot each are more than
+    the persistent grid's segments, so the segments take the slots by ticket.
+    Every search must run, and every frame equal the oracle."""
+    make, keep = locator()
+    sc = S.config4(make)
+    print(f"config4 full batched {sc.n} agents", flush=True)
+    gs, xy, st = run_pair(oracle, sc, G.MODE_BATCHED, 256, frames=120, check_every=30, hash_cell=4.0)
+    assert gs["found"] > 0.9 * sc.n
+    assert gs["frame_kinds"]["crowd_only"] > 0
 
 
 def test_config3_maze_full_corridors_are_tree_paths(oracle):

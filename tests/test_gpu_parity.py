@@ -142,7 +142,7 @@ def test_world_parity_on_one_way_neighbour_lists(oracle, hub, mode, budget):
     assert gs["found"] > 50
 
 
-def _world(seed, nx, ny, n_agents, T, Q, mode=G.MODE_PLAIN, budget=0, p=0.2, P=None, spw=0, wide=False):
+def _world(seed, nx, ny, n_agents, T, Q, mode=G.MODE_PLAIN, budget=0, p=0.2, P=None, spw=0, wide=False, wg=0):
     tri, nbr = H.grid_map(seed, nx, ny, cell=4.0, p=p)
     cen = S.centroids(tri)
     lab = S.components(nbr)
@@ -155,27 +155,31 @@ def _world(seed, nx, ny, n_agents, T, Q, mode=G.MODE_PLAIN, budget=0, p=0.2, P=N
     cfg = G.Config(agents_max_count=n_agents, path_finder_parallel_agents=P or n_agents,
                    path_finder_threads_per_agents=T, path_finder_queue_capacity=Q,
                    agent_radius=0.5, agent_speed=2.0, step_mode=mode, round_budget=budget or 1,
-                   searches_per_wave=spw, search_wide_keys=wide)
+                   searches_per_wave=spw, search_wide_keys=wide, search_workgroups=wg)
     return tri, nbr, cfg, starts, targs
 
 
-@pytest.mark.parametrize("mode,budget,P,spw,T,Q,wide", [(G.MODE_PLAIN, 0, None, 0, 8, 16, False),
-                                                        (G.MODE_PLAIN, 0, 37, 0, 8, 16, False),
-                                                        (G.MODE_BATCHED, 3, None, 0, 8, 16, False),
-                                                        (G.MODE_BATCHED, 1, 50, 0, 8, 16, False),
-                                                        (G.MODE_BATCHED, 3, None, 3, 8, 16, False),
+@pytest.mark.parametrize("mode,budget,P,spw,T,Q,wide,wg", [(G.MODE_PLAIN, 0, None, 0, 8, 16, False, 0),
+                                                        (G.MODE_PLAIN, 0, 37, 0, 8, 16, False, 0),
+                                                        (G.MODE_BATCHED, 3, None, 0, 8, 16, False, 0),
+                                                        (G.MODE_BATCHED, 1, 50, 0, 8, 16, False, 0),
+                                                        (G.MODE_BATCHED, 3, None, 3, 8, 16, False, 0),
                                                         # queues parked and restored past 16 items
-                                                        (G.MODE_BATCHED, 2, None, 0, 4, 64, False),
-                                                        (G.MODE_BATCHED, 3, 60, 0, 2, 32, False),
+                                                        (G.MODE_BATCHED, 2, None, 0, 4, 64, False, 0),
+                                                        (G.MODE_BATCHED, 3, 60, 0, 2, 32, False, 0),
                                                         # keys without G and side, parked and restored
-                                                        (G.MODE_BATCHED, 3, None, 0, 4, 16, True),
-                                                        (G.MODE_BATCHED, 2, 60, 0, 2, 32, True),
-                                                        (G.MODE_PLAIN, 0, 37, 0, 4, 16, True)])
-def test_multi_frame_world_parity(oracle, mode, budget, P, spw, T, Q, wide):
+                                                        (G.MODE_BATCHED, 3, None, 0, 4, 16, True, 0),
+                                                        (G.MODE_BATCHED, 2, 60, 0, 2, 32, True, 0),
+                                                        (G.MODE_PLAIN, 0, 37, 0, 4, 16, True, 0),
+                                                        # a slot per agent, 24 and 32 segments for 160
+                                                        # agents: the segments take the slots by ticket
+                                                        (G.MODE_BATCHED, 3, None, 0, 8, 16, False, 3),
+                                                        (G.MODE_BATCHED, 2, None, 0, 4, 16, False, 2)])
+def test_multi_frame_world_parity(oracle, mode, budget, P, spw, T, Q, wide, wg):
     """Agents search, move, avoid and progress for many frames; every frame
     must equal the oracle bit for bit (positions, states, nodes, corridors)."""
-    tri, nbr, cfg, starts, targs = _world(21, 40, 40, 160, T, Q, mode, budget, P=P, spw=spw, wide=wide)
-    print(f"world parity mode={mode} budget={budget} P={P} spw={spw} T={T} Q={Q} wide={wide}", flush=True)
+    tri, nbr, cfg, starts, targs = _world(21, 40, 40, 160, T, Q, mode, budget, P=P, spw=spw, wide=wide, wg=wg)
+    print(f"world parity mode={mode} budget={budget} P={P} spw={spw} T={T} Q={Q} wide={wide} wg={wg}", flush=True)
     g = H.make_gpu(tri, nbr, cfg, starts, targs)
     w = H.make_oracle(oracle, tri, nbr, cfg, starts, targs)
     dt = 1.0 / 60
@@ -231,6 +235,41 @@ def test_target_updates_made_during_a_step_apply_at_the_next(oracle, mode, budge
     assert gs["found"] > 0
     for k in ("found", "not_found", "expansions", "discoveries", "corridor_nodes", "rounds", "admitted"):
         assert gs[k] == os_[k], (k, gs[k], os_[k])
+    g.close()
+
+
+@pytest.mark.parametrize("mode,budget", [(G.MODE_PLAIN, 0), (G.MODE_BATCHED, 2)])
+def test_crowd_only_frames_and_back(oracle, mode, budget):
+    """Once no agent searches and no target changes, a frame runs without
+    admission and search (the moving set, hash, avoidance, progress and the
+    readback only); a target change brings the full frame back.  Every frame
+    equals the oracle bit for bit, through both transitions."""
+    tri, nbr, cfg, starts, targs = _world(41, 36, 36, 120, 4, 16, mode, budget)
+    print(f"crowd-only frames mode={mode} budget={budget}", flush=True)
+    g = H.make_gpu(tri, nbr, cfg, starts, targs)
+    w = H.make_oracle(oracle, tri, nbr, cfg, starts, targs)
+    dt = 1.0 / 60
+    seen_crowd_only = 0
+    for f in range(160):
+        before = g.frame_kinds()["crowd_only"]
+        g.step(dt)
+        w.step(dt)
+        H.compare_worlds(g, w, f, check_corridors=(f % 10 == 0))
+        if g.frame_kinds()["crowd_only"] > before:
+            seen_crowd_only += 1
+        if f == 100:  # searches again: the next frame is a full one
+            assert seen_crowd_only > 0
+            k0 = g.frame_kinds()["crowd_only"]
+            for i in range(0, 120, 9):
+                x, y = targs[(i * 7) % 120]
+                g.set_agent_targ_pos(i, x, y)
+                assert w.set_target(i, x, y) == 0
+        if f == 101:
+            assert g.frame_kinds()["crowd_only"] == k0  # frame 101 admitted and searched
+    gs, os_ = g.stats(), w.stats()
+    for k in ("found", "not_found", "expansions", "discoveries", "corridor_nodes", "rounds", "admitted"):
+        assert gs[k] == os_[k], (k, gs[k], os_[k])
+    assert gs["admitted"] > 120
     g.close()
 
 

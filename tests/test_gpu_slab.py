@@ -41,12 +41,14 @@ def _run(tri, nbr, starts, targets, mode, slab, frames=800, P=0, wg=0):
     return first, seen_move_frame, info, stats
 
 
-@pytest.mark.parametrize("mode,P,wg", [(G.MODE_PLAIN, 0, 0), (G.MODE_BATCHED, 0, 0), (G.MODE_BATCHED, 60, 2)])
+@pytest.mark.parametrize("mode,P,wg", [(G.MODE_PLAIN, 0, 0), (G.MODE_BATCHED, 0, 0), (G.MODE_BATCHED, 60, 2),
+                                      (G.MODE_BATCHED, 0, 2)])
 def test_small_slab_defers_and_compacts_without_changing_corridors(oracle, mode, P, wg):
     """P, wg: fewer slots than agents and a persistent grid of wg workgroups
     (32 segments for 60 slots), so the workgroups take searches by atomic
     ticket and a segment whose deferred corridor still does not fit takes
-    the next ticket."""
+    the next ticket.  P = 0 with wg: a slot per agent and 32 segments for
+    160 slots, taken by ticket too."""
     tri, nbr = H.grid_map(11, 24, 24, cell=4.0, p=0.15)
     pairs = H.connected_pairs(12, nbr, 160)
     cen = tri.reshape(-1, 3, 2).mean(axis=1).astype(np.float32)
