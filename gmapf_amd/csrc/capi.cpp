@@ -121,8 +121,9 @@ struct gmapf_s {
     int* h_state_stage = nullptr;
     unsigned long long* h_stats = nullptr;
     unsigned long long* h_slab = nullptr;  // [0] slab ints in use, [1] deferred corridors
-    int* h_search = nullptr;  // agents in Search after the last frame, per 64 agents
-    int searching_after = -1;  // their sum (-1: no frame ran yet)
+    int* h_group = nullptr;  // agents in Search and in Move after the last frame, per 64 agents
+    int searching_after = -1;  // their sums (-1: no frame ran yet)
+    int moving_after = -1;
     long long crowd_frames = 0;  // frames that ran without admission and search
     int* h_upd = nullptr;       // [0] count, then ids[N], targ[N]
     float2* h_upd_xy = nullptr;
@@ -198,9 +199,10 @@ struct gmapf_s {
         if (h_state_stage) hipHostFree(h_state_stage);
         if (h_stats) hipHostFree(h_stats);
         if (h_slab) hipHostFree(h_slab);
-        if (h_search) hipHostFree(h_search);
-        h_search = nullptr;
+        if (h_group) hipHostFree(h_group);
+        h_group = nullptr;
         searching_after = -1;
+        moving_after = -1;
         if (h_upd) hipHostFree(h_upd);
         if (h_upd_xy) hipHostFree(h_upd_xy);
         h_step = h_pos_stage = nullptr;
@@ -502,9 +504,10 @@ struct gmapf_s {
         HIPCHK(hipHostMalloc((void**)&h_stats, 8 * sizeof(unsigned long long)));
         HIPCHK(hipHostMalloc((void**)&h_slab, 2 * sizeof(unsigned long long)));
         h_slab[0] = h_slab[1] = 0;
-        HIPCHK(hipHostMalloc((void**)&h_search, (size_t)std::max(1, (N + 63) / 64) * sizeof(int)));
-        std::memset(h_search, 0, (size_t)std::max(1, (N + 63) / 64) * sizeof(int));
+        HIPCHK(hipHostMalloc((void**)&h_group, (size_t)std::max(1, (N + 63) / 64) * sizeof(int)));
+        std::memset(h_group, 0, (size_t)std::max(1, (N + 63) / 64) * sizeof(int));
         searching_after = -1;
+        moving_after = -1;
         upd_cap = std::max(N, 1);
         HIPCHK(hipHostMalloc((void**)&h_upd, (1 + 2 * (size_t)upd_cap) * sizeof(int)));
         HIPCHK(hipHostMalloc((void**)&h_upd_xy, (size_t)upd_cap * sizeof(float2)));
@@ -523,7 +526,7 @@ struct gmapf_s {
         dfr.h_state_out = h_state_stage;
         dfr.h_stats_out = h_stats;
         dfr.h_slab_out = h_slab;
-        dfr.h_search_out = h_search;
+        dfr.h_group_out = h_group;
 
         // ---- search launch shape
         const int rpw = cfg.searches_per_wave > 0 ? cfg.searches_per_wave : 64 / T;
@@ -595,14 +598,11 @@ struct gmapf_s {
         HIPCHK(launch_export(dag, dfr, s1));
     }
 
-    // A frame after which no agent searched, with no target change since:
-    // admission and the search have nothing to do, so the frame is the moving
-    // set, the crowd phase and the export, on one stream.
-    void enqueue_crowd_frame() {
-        HIPCHK(launch_crowd_start(dag, dfr, s1));
-        HIPCHK(launch_crowd(dmap, dag, dfr, cp, s1));
-        HIPCHK(launch_export(dag, dfr, s1));
-    }
+    // A frame after which no agent searched and some moved, with no target
+    // change since: admission and the search have nothing to do, so the frame
+    // is the crowd phase alone, on one stream (its move pass takes the moving
+    // set, its last pass exports).
+    void enqueue_crowd_frame() { HIPCHK(launch_crowd(dmap, dag, dfr, cp, s1, true)); }
 
     void build_crowd_graph() {
         HIPCHK(hipStreamBeginCapture(s1, hipStreamCaptureModeThreadLocal));
@@ -654,8 +654,11 @@ struct gmapf_s {
         // last frame exported; no device round trip)
         const bool compact = compact_next;
         // nothing to admit or search: no agent searched after the last frame
-        // and no target changed since (a compaction waits for a full frame)
-        const bool crowd_only = searching_after == 0 && upd_n == 0 && !compact && !cfg.is_profiler && !cfg.is_debug;
+        // and no target changed since (a compaction waits for a full frame);
+        // a frame in which nobody moves either runs in full, where every
+        // pass finds nothing to do
+        const bool crowd_only = searching_after == 0 && moving_after > 0 && upd_n == 0 && !compact &&
+                                !cfg.is_profiler && !cfg.is_debug;
         if (compact) compactions++;
         const bool timed = !crowd_only && cfg.search_timing > 0 && frames % cfg.search_timing == 0;
         step_timed = timed;
@@ -706,7 +709,11 @@ struct gmapf_s {
         slab_in_use = (long long)h_slab[0];
         slab_deferred = (long long)h_slab[1];
         searching_after = 0;
-        for (int w = 0; w < (n_agents + 63) / 64; w++) searching_after += h_search[w];
+        moving_after = 0;
+        for (int w = 0; w < (n_agents + 63) / 64; w++) {
+            searching_after += h_group[w] & 0xff;
+            moving_after += (h_group[w] >> 8) & 0xff;
+        }
         compact_next = 2 * slab_in_use > dag.slab_cap;
         frames++;
         stats_host[7] = frames;

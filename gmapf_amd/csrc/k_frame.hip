@@ -469,40 +469,30 @@ __global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
 // store there inside the round loop would be waited for by the next round's
 // wait for all memory operations (the counter is in order), and the stores
 // would also hold up the end of the launch.
-__global__ void k_export(DevAgents ag, DevFrame fr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Thread i of a grid of at least max(N, 8) threads, in whole waves.
+__device__ __forceinline__ void export_agent(const DevAgents& ag, const DevFrame& fr, int i) {
     const int s = i < ag.n ? ag.state[i] : 0;
     const bool changed = i < ag.n && s != fr.state_prev[i];
     if (__ballot(changed) != 0ull && i < ag.n) {
         fr.h_state_out[i] = s;
         if (changed) fr.state_prev[i] = s;
     }
-    // agents left searching, per 64-agent group (the host adds them up: a
-    // frame after which none searches and before which no target changed
-    // needs no admission or search)
+    // agents left searching and moving, per 64-agent group (the host adds
+    // them up: a frame after which none searches and some move, and before
+    // which no target changed, needs no admission or search)
     const unsigned long long srch = __ballot(i < ag.n && s == kSearch);
-    if ((threadIdx.x & 63) == 0 && i < ag.n) fr.h_search_out[i >> 6] = __popcll(srch);
+    const unsigned long long mov = __ballot(i < ag.n && s == kMove);
+    if ((threadIdx.x & 63) == 0 && i < ag.n) fr.h_group_out[i >> 6] = __popcll(srch) | (__popcll(mov) << 8);
     if (i < 8) fr.h_stats_out[i] = fr.stats[i];
     if (i < 2) fr.h_slab_out[i] = ag.slab_top[i];  // [0] top, [1] deferred (adjacent words)
     // the movers of this frame are counted; the next frame counts from 0
-    // (its admission pass, or its crowd-only start, adds them up again)
+    // (its admission pass, or the move pass of a frame without one, adds
+    // them up again)
     if (i == 0) fr.counters[5] = 0;
 }
 
-// First kernel of a crowd-only frame (no agent searches and no target
-// changed since the last frame, so admission and the search have nothing to
-// do): the frame's step and its moving set, which admission takes otherwise
-// (the movers are counted from the 0 the last frame's export left).
-__global__ void k_crowd_start(DevAgents ag, DevFrame fr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        *fr.stepv = *fr.h_step;
-        fr.counters[6] = 0;
-    }
-    const bool mv = i < ag.n && ag.state[i] == kMove;
-    if (i < ag.n) ag.in_m[i] = mv ? 1 : 0;
-    const unsigned long long b = __ballot(mv);
-    if ((threadIdx.x & 63) == 0 && b != 0ull) atomicAdd(&fr.counters[5], (int)__popcll(b));
+__global__ void k_export(DevAgents ag, DevFrame fr) {
+    export_agent(ag, fr, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ------------------------------------------------------------- corridor slab
@@ -572,13 +562,28 @@ __global__ void k_slab_finish(DevAgents ag, DevSlots sl, DevFrame fr) {
 //                  members with a smaller index: its rank, and its place in
 //                  the final order (horder)
 // AgentsMover.h:57-70 (move), then the agent's hash cell from its moved
-// position, counted into hcount (all zero at the start of a frame)
+// position, counted into hcount (all zero at the start of a frame).
+// kAlone: a frame without admission, so this pass takes the moving set (the
+// agents in Move; the host runs such a frame only when some agent moves) and
+// counts it from the 0 the last frame's export left, and reads the step from
+// the host.
+template <bool kAlone>
 __global__ void k_move_key(DevMap map, DevAgents ag, DevFrame fr) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ag.n || fr.counters[5] == 0) return;
+    bool mv;
+    if constexpr (kAlone) {
+        mv = i < ag.n && ag.state[i] == kMove;
+        const unsigned long long b = __ballot(mv);
+        if ((threadIdx.x & 63) == 0 && b != 0ull) atomicAdd(&fr.counters[5], (int)__popcll(b));
+        if (i >= ag.n) return;
+        ag.in_m[i] = mv ? 1 : 0;
+    } else {
+        if (i >= ag.n || fr.counters[5] == 0) return;
+        mv = ag.in_m[i] != 0;
+    }
     float2 p = ag.pos[i];
-    if (ag.in_m[i] && ag.step_idx[i] != -1) {
-        const float step = *fr.stepv;
+    if (mv && ag.step_idx[i] != -1) {
+        const float step = kAlone ? *fr.h_step : *fr.stepv;
         const float2 sp = ag.step_pos[i];
         float dx = sp.x - p.x, dy = sp.y - p.y;
         float len = sqrtf(dx * dx + dy * dy);
@@ -833,10 +838,18 @@ __global__ __launch_bounds__(kAvoidThreads) void k_avoid_progress(DevMap map, De
 
 // The frame's positions into pinned host memory in agent order (the host
 // writes stay whole lines), after the avoidance pass has written them all.
+// kExport: a frame without a search, whose export this pass also does (it
+// clears the movers' count, so it does not test it: such a frame has movers).
+template <bool kExport>
 __global__ void k_pos_out(DevAgents ag, DevFrame fr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ag.n || fr.counters[5] == 0) return;
-    fr.h_pos_out[i] = ag.pos[i];
+    if constexpr (kExport) {
+        if (i < ag.n) fr.h_pos_out[i] = ag.pos[i];
+        export_agent(ag, fr, i);
+    } else {
+        if (i >= ag.n || fr.counters[5] == 0) return;
+        fr.h_pos_out[i] = ag.pos[i];
+    }
 }
 
 // Staging for bulk point location without allocation: the points, then the
@@ -889,11 +902,6 @@ hipError_t launch_export(const DevAgents& ag, const DevFrame& fr, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t launch_crowd_start(const DevAgents& ag, const DevFrame& fr, hipStream_t st) {
-    hipLaunchKernelGGL(k_crowd_start, dim3(nblk(ag.n > 0 ? ag.n : 1, 256)), dim3(256), 0, st, ag, fr);
-    return hipGetLastError();
-}
-
 hipError_t launch_compact(const DevAgents& ag, const DevSlots& sl, const DevFrame& fr, hipStream_t st) {
     const int owners = ag.n + sl.P;
     if (ag.n <= 0) return hipSuccess;
@@ -908,9 +916,12 @@ hipError_t launch_compact(const DevAgents& ag, const DevSlots& sl, const DevFram
 }
 
 hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& fr,
-                        const CrowdParams& cp, hipStream_t st) {
+                        const CrowdParams& cp, hipStream_t st, bool alone) {
     const int nb = nblk(ag.n, 256);
-    hipLaunchKernelGGL(k_move_key, dim3(nb), dim3(256), 0, st, map, ag, fr);
+    if (alone)
+        hipLaunchKernelGGL(k_move_key<true>, dim3(nb), dim3(256), 0, st, map, ag, fr);
+    else
+        hipLaunchKernelGGL(k_move_key<false>, dim3(nb), dim3(256), 0, st, map, ag, fr);
     // the scan reads and zeroes the counts; nothing in a frame with no mover
     hipError_t e;
     if ((e = exscan(fr.hcount, fr.ncell + 1, fr.hblk, fr.hstart, &fr.hstart[fr.ncell + 1], st, fr.hcount,
@@ -919,7 +930,10 @@ hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& 
     hipLaunchKernelGGL(k_hash_place, dim3(nb), dim3(256), 0, st, ag, fr);
     hipLaunchKernelGGL(k_avoid_progress, dim3(nblk(ag.n, kAvoidThreads)), dim3(kAvoidThreads), 0, st, map, ag, fr,
                        cp);
-    hipLaunchKernelGGL(k_pos_out, dim3(nb), dim3(256), 0, st, ag, fr);
+    if (alone)
+        hipLaunchKernelGGL(k_pos_out<true>, dim3(nblk(ag.n > 8 ? ag.n : 8, 256)), dim3(256), 0, st, ag, fr);
+    else
+        hipLaunchKernelGGL(k_pos_out<false>, dim3(nb), dim3(256), 0, st, ag, fr);
     return hipGetLastError();
 }
 

@@ -27,15 +27,15 @@ hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& 
                         bool ident, hipStream_t st);
 // positions, states and counters into pinned host memory
 hipError_t launch_export(const DevAgents& ag, const DevFrame& fr, hipStream_t st);
-// first kernel of a frame in which nothing is admitted or searched
-hipError_t launch_crowd_start(const DevAgents& ag, const DevFrame& fr, hipStream_t st);
 // corridor slab compaction: corridors of moving agents into the other buffer
 // in agent order; the others are released
 hipError_t launch_compact(const DevAgents& ag, const DevSlots& sl, const DevFrame& fr, hipStream_t st);
 // F4: move, hash + avoidance (Jacobi), progress; all no-ops when the moving
-// set of the frame is empty.
+// set of the frame is empty.  alone: the frame has no admission and no search
+// (nobody searches, no target changed, some agent moves): the first kernel
+// takes the frame's step and moving set and the last one also exports.
 hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& fr,
-                        const CrowdParams& cp, hipStream_t st);
+                        const CrowdParams& cp, hipStream_t st, bool alone = false);
 
 // bulk point location (min containing triangle or -1)
 hipError_t launch_locate(const DevMap& map, const float2* pts, int n, int* out, hipStream_t st);

@@ -168,10 +168,10 @@ int gmapf_last_frame_ms(gmapf_handle h, double* out8);
 int gmapf_get_hash(gmapf_handle h, int* keys, int* order, int* n);
 /* resolved build knobs: [0] T [1] Q [2] P [3] budget [4] search grid [5] mode [6] V */
 int gmapf_get_launch_info(gmapf_handle h, int* out8);
-/* frames so far: [0] all, [1] those that ran without admission and search (no agent
- * searched after the frame before and no target changed since: the frame is the
- * move, the spatial hash, the avoidance, progress and the readback); no reference
- * counterpart */
+/* frames so far: [0] all, [1] those that ran without admission and search (after the
+ * frame before no agent searched and some moved, and no target changed since: the
+ * frame is the move, the spatial hash, the avoidance, progress and the readback);
+ * no reference counterpart */
 int gmapf_get_frame_kinds(gmapf_handle h, long long* out2);
 /* corridor slab after the last frame: [0] capacity (ints per buffer) [1] ints in use
  * [2] compactions so far [3] corridors deferred for lack of space so far */
