@@ -124,6 +124,10 @@ struct gmapf_s {
     int* h_group = nullptr;  // agents in Search and in Move after the last frame, per 64 agents
     int searching_after = -1;  // their sums (-1: no frame ran yet)
     int moving_after = -1;
+    // 64-agent groups whose mirrored states the host changed (a target set):
+    // the next readback copies them whatever the device exported
+    std::vector<int> host_dirty;
+    std::vector<unsigned char> host_dirty_flag;
     long long crowd_frames = 0;  // frames that ran without admission and search
     int* h_upd = nullptr;       // [0] count, then ids[N], targ[N]
     float2* h_upd_xy = nullptr;
@@ -203,6 +207,8 @@ struct gmapf_s {
         h_group = nullptr;
         searching_after = -1;
         moving_after = -1;
+        host_dirty.clear();
+        host_dirty_flag.clear();
         if (h_upd) hipHostFree(h_upd);
         if (h_upd_xy) hipHostFree(h_upd_xy);
         h_step = h_pos_stage = nullptr;
@@ -701,19 +707,25 @@ struct gmapf_s {
         HIPCHK(q);
         HIPCHK(hipGetLastError());
         const Clock::time_point t_sync = Clock::now();
-        if (n_agents > 0) {
-            std::memcpy(pos.data(), h_pos_stage, (size_t)n_agents * sizeof(float2));
-            std::memcpy(state.data(), h_state_stage, (size_t)n_agents * sizeof(int));
-        }
-        for (int k = 0; k < 8; k++) stats_host[k] = (long long)h_stats[k];
-        slab_in_use = (long long)h_slab[0];
-        slab_deferred = (long long)h_slab[1];
+        if (n_agents > 0) std::memcpy(pos.data(), h_pos_stage, (size_t)n_agents * sizeof(float2));
+        // states: the groups the export rewrote (it writes a group whole when
+        // one of its states changed) and those whose mirror the host changed
         searching_after = 0;
         moving_after = 0;
         for (int w = 0; w < (n_agents + 63) / 64; w++) {
-            searching_after += h_group[w] & 0xff;
-            moving_after += (h_group[w] >> 8) & 0xff;
+            const int word = h_group[w];
+            searching_after += word & 0xff;
+            moving_after += (word >> 8) & 0xff;
+            if ((word >> 16) & 1 || (w < (int)host_dirty_flag.size() && host_dirty_flag[w])) {
+                const int b = 64 * w, e = std::min(n_agents, b + 64);
+                std::memcpy(state.data() + b, h_state_stage + b, (size_t)(e - b) * sizeof(int));
+            }
         }
+        for (const int w : host_dirty) host_dirty_flag[w] = 0;
+        host_dirty.clear();
+        for (int k = 0; k < 8; k++) stats_host[k] = (long long)h_stats[k];
+        slab_in_use = (long long)h_slab[0];
+        slab_deferred = (long long)h_slab[1];
         compact_next = 2 * slab_in_use > dag.slab_cap;
         frames++;
         stats_host[7] = frames;
@@ -724,6 +736,7 @@ struct gmapf_s {
         for (size_t k = 0; k < upd_ids.size(); k++) {
             const int id = upd_ids[k];
             state[id] = GMAPF_STATE_SEARCH;
+            mark_host_dirty(id);
             stage_update(id, upd_targ[k], upd_xy[2 * k], upd_xy[2 * k + 1]);
             upd_qpos[id] = -1;
         }
@@ -844,6 +857,15 @@ struct gmapf_s {
         h_upd_xy[k] = make_float2(x, y);
     }
 
+    void mark_host_dirty(int id) {
+        const int w = id >> 6;
+        if ((int)host_dirty_flag.size() <= w) host_dirty_flag.resize((size_t)w + 1, 0);
+        if (!host_dirty_flag[w]) {
+            host_dirty_flag[w] = 1;
+            host_dirty.push_back(w);
+        }
+    }
+
     void set_target(int id, float x, float y) {
         require_init();
         if (!map_filled) throw ApiError(GMAPF_E_STATE, "map is not filled");
@@ -852,6 +874,7 @@ struct gmapf_s {
         if (node < 0)
             throw ApiError(GMAPF_E_POS, "target pos (" + std::to_string(x) + ", " + std::to_string(y) + ") is invalid");
         state[id] = GMAPF_STATE_SEARCH;
+        mark_host_dirty(id);
         if (!started) {
             targ[id] = node;
             targ_pos[2 * id] = x;

@@ -148,7 +148,8 @@ struct DevFrame {
     unsigned long long* h_stats_out;  // [8] cumulative counters
     unsigned long long* h_slab_out;   // [2] slab top, deferred corridors
     int* h_group_out;                 // [ceil(N / 64)] per 64 agents after the frame: agents in
-                                      // Search (bits 0-7) and in Move (bits 8-15)
+                                      // Search (bits 0-7) and in Move (bits 8-15), and bit 16
+                                      // when the group's states in h_state_out were rewritten
     // corridor slab compaction scratch
     int* slab_len;                    // N + 1: live corridor lengths, then their offsets
     int* slab_blk;                    // scan partials

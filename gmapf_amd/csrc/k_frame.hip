@@ -479,10 +479,13 @@ __device__ __forceinline__ void export_agent(const DevAgents& ag, const DevFrame
     }
     // agents left searching and moving, per 64-agent group (the host adds
     // them up: a frame after which none searches and some move, and before
-    // which no target changed, needs no admission or search)
+    // which no target changed, needs no admission or search), and whether
+    // the group's states were written (the host copies only those groups)
     const unsigned long long srch = __ballot(i < ag.n && s == kSearch);
     const unsigned long long mov = __ballot(i < ag.n && s == kMove);
-    if ((threadIdx.x & 63) == 0 && i < ag.n) fr.h_group_out[i >> 6] = __popcll(srch) | (__popcll(mov) << 8);
+    const unsigned long long chg = __ballot(changed);
+    if ((threadIdx.x & 63) == 0 && i < ag.n)
+        fr.h_group_out[i >> 6] = __popcll(srch) | (__popcll(mov) << 8) | (chg != 0ull ? 1 << 16 : 0);
     if (i < 8) fr.h_stats_out[i] = fr.stats[i];
     if (i < 2) fr.h_slab_out[i] = ag.slab_top[i];  // [0] top, [1] deferred (adjacent words)
     // the movers of this frame are counted; the next frame counts from 0
