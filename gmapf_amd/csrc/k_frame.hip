@@ -568,8 +568,9 @@ __global__ void k_slab_finish(DevAgents ag, DevSlots sl, DevFrame fr) {
 // position, counted into hcount (all zero at the start of a frame).
 // kAlone: a frame without admission, so this pass takes the moving set (the
 // agents in Move; the host runs such a frame only when some agent moves) and
-// counts it from the 0 the last frame's export left, and reads the step from
-// the host.
+// counts it from the 0 the last frame's export left.  (Its step is in device
+// memory: k_set_step puts it there when it changed.  Read from pinned host
+// memory by every wave, it made this pass 2.7x longer.)
 template <bool kAlone>
 __global__ void k_move_key(DevMap map, DevAgents ag, DevFrame fr) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -586,7 +587,7 @@ __global__ void k_move_key(DevMap map, DevAgents ag, DevFrame fr) {
     }
     float2 p = ag.pos[i];
     if (mv && ag.step_idx[i] != -1) {
-        const float step = kAlone ? *fr.h_step : *fr.stepv;
+        const float step = *fr.stepv;
         const float2 sp = ag.step_pos[i];
         float dx = sp.x - p.x, dy = sp.y - p.y;
         float len = sqrtf(dx * dx + dy * dy);
@@ -839,6 +840,11 @@ __global__ __launch_bounds__(kAvoidThreads) void k_avoid_progress(DevMap map, De
     }
 }
 
+// The step of a frame without admission, when it differs from the last frame's.
+__global__ void k_set_step(DevFrame fr) {
+    if (threadIdx.x == 0) *fr.stepv = *fr.h_step;
+}
+
 // The frame's positions into pinned host memory in agent order (the host
 // writes stay whole lines), after the avoidance pass has written them all.
 // kExport: a frame without a search, whose export this pass also does (it
@@ -919,8 +925,9 @@ hipError_t launch_compact(const DevAgents& ag, const DevSlots& sl, const DevFram
 }
 
 hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& fr,
-                        const CrowdParams& cp, hipStream_t st, bool alone) {
+                        const CrowdParams& cp, hipStream_t st, bool alone, bool new_step) {
     const int nb = nblk(ag.n, 256);
+    if (alone && new_step) hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, st, fr);
     if (alone)
         hipLaunchKernelGGL(k_move_key<true>, dim3(nb), dim3(256), 0, st, map, ag, fr);
     else

@@ -33,9 +33,10 @@ hipError_t launch_compact(const DevAgents& ag, const DevSlots& sl, const DevFram
 // F4: move, hash + avoidance (Jacobi), progress; all no-ops when the moving
 // set of the frame is empty.  alone: the frame has no admission and no search
 // (nobody searches, no target changed, some agent moves): the first kernel
-// takes the frame's step and moving set and the last one also exports.
+// takes the frame's moving set and the last one also exports; new_step (with
+// alone): the step differs from the last frame's and is copied first.
 hipError_t launch_crowd(const DevMap& map, const DevAgents& ag, const DevFrame& fr,
-                        const CrowdParams& cp, hipStream_t st, bool alone = false);
+                        const CrowdParams& cp, hipStream_t st, bool alone = false, bool new_step = false);
 
 // bulk point location (min containing triangle or -1)
 hipError_t launch_locate(const DevMap& map, const float2* pts, int n, int* out, hipStream_t st);

@@ -243,14 +243,15 @@ def test_crowd_only_frames_and_back(oracle, mode, budget):
     """Once no agent searches and no target changes, a frame runs without
     admission and search (the moving set, hash, avoidance, progress and the
     readback only); a target change brings the full frame back.  Every frame
-    equals the oracle bit for bit, through both transitions."""
+    equals the oracle bit for bit, through both transitions and through
+    changes of the time step (every 25th frame is twice as long)."""
     tri, nbr, cfg, starts, targs = _world(41, 36, 36, 120, 4, 16, mode, budget)
     print(f"crowd-only frames mode={mode} budget={budget}", flush=True)
     g = H.make_gpu(tri, nbr, cfg, starts, targs)
     w = H.make_oracle(oracle, tri, nbr, cfg, starts, targs)
-    dt = 1.0 / 60
     seen_crowd_only = 0
     for f in range(160):
+        dt = 1.0 / 30 if f % 25 == 0 else 1.0 / 60
         before = g.frame_kinds()["crowd_only"]
         g.step(dt)
         w.step(dt)
