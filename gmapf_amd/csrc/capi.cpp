@@ -477,6 +477,7 @@ struct gmapf_s {
         dfr.active = dalloc<int>(P, "active slots");
         dfr.counters = dalloc<int>(16, "counters");
         dfr.stats = dalloc<unsigned long long>(8, "stats");
+        dfr.stats_part = dalloc<unsigned long long>(8 * kStatStripes, "striped frame stats");
         dfr.ncell = dmap.hnx * dmap.hny;
         dfr.hkey = dalloc<int>(Nn, "hash keys");
         dfr.hcount = dalloc<int>((size_t)dfr.ncell + 1, "hash counts");
@@ -495,6 +496,7 @@ struct gmapf_s {
         if (N > 0) HIPCHK(hipMemcpy(dfr.state_prev, state.data(), N * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(launch_fill(dfr.counters, 0, 16, s1));
         HIPCHK(launch_fill_u64(dfr.stats, 0, 8, s1));
+        HIPCHK(launch_fill_u64(dfr.stats_part, 0, 8 * kStatStripes, s1));
         HIPCHK(launch_fill(dfr.hkey, -1, Nn, s1));
 
         // ---- pinned staging

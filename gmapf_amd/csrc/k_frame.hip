@@ -370,8 +370,8 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_scatter(DevAgents ag, D
 // Admission when every agent has a slot of its own (P >= N): every request
 // is admitted (PathFinder.h:85-116 with no deferral), into slot i for agent i
 // (PathFinder.h:104-115); the moving set is taken as in k_admit_count.  One
-// pass, no ranks; the counts go to counters[5], counters[6] (cleared by
-// k_apply_updates) and stats[6].
+// pass, no ranks; counters[5] (cleared by k_apply_updates) is set when some
+// agent moves, and the admissions are added to stats[6] through its stripes.
 __global__ __launch_bounds__(kAdmitThreads) void k_admit_identity(DevAgents ag, DevSlots sl, DevFrame fr) {
     __shared__ int wsum_m[kAdmitWaves], wsum_a[kAdmitWaves];
     const int i = blockIdx.x * kAdmitThreads + threadIdx.x;
@@ -399,12 +399,12 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_identity(DevAgents ag, 
     int nm, na;
     admit_rank(mv, wsum_m, &nm);
     admit_rank(adm, wsum_a, &na);
+    // the movers' count is only tested against 0, and nothing reads the
+    // admitted count of this form: a flag and a striped add, no atomics on
+    // one word from every block
     if (threadIdx.x == 0) {
-        if (nm) atomicAdd(&fr.counters[5], nm);
-        if (na) {
-            atomicAdd(&fr.counters[6], na);
-            atomicAdd(&fr.stats[6], (unsigned long long)na);
-        }
+        if (nm) fr.counters[5] = 1;
+        if (na) atomicAdd(&fr.stats_part[(blockIdx.x % kStatStripes) * 8 + 6], (unsigned long long)na);
     }
 }
 
@@ -486,7 +486,18 @@ __device__ __forceinline__ void export_agent(const DevAgents& ag, const DevFrame
     const unsigned long long chg = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && i < ag.n)
         fr.h_group_out[i >> 6] = __popcll(srch) | (__popcll(mov) << 8) | (chg != 0ull ? 1 << 16 : 0);
-    if (i < 8) fr.h_stats_out[i] = fr.stats[i];
+    if (i < 8) {
+        // the frame's additions, striped over kStatStripes sets of words,
+        // into the cumulative counters (and the stripes back to 0)
+        unsigned long long add = 0;
+#pragma unroll 16
+        for (int p = 0; p < kStatStripes; ++p) add += fr.stats_part[p * 8 + i];
+#pragma unroll 16
+        for (int p = 0; p < kStatStripes; ++p) fr.stats_part[p * 8 + i] = 0ull;
+        const unsigned long long t = fr.stats[i] + add;
+        fr.stats[i] = t;
+        fr.h_stats_out[i] = t;
+    }
     if (i < 2) fr.h_slab_out[i] = ag.slab_top[i];  // [0] top, [1] deferred (adjacent words)
     // the movers of this frame are counted; the next frame counts from 0
     // (its admission pass, or the move pass of a frame without one, adds

@@ -554,7 +554,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     unsigned pf_vn[3] = {0u, 0u, 0u};
     // corridor walk of this lane's side of the segment's last found search
     Walk walk;
-    // frame statistics, added to fr.stats once per workgroup at the end
+    // frame statistics, added to a stripe of fr.stats_part once per
+    // workgroup at the end
     unsigned st_found = 0, st_fail = 0, st_exp = 0, st_disc = 0, st_corr = 0, st_rounds = 0;
 
     // Acquisition and the test for the end of the workgroup run only when a
@@ -1053,12 +1054,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         st_rounds += __shfl_xor(st_rounds, o, kWave);
     }
     if (L == 0) {
-        if (st_found) atomicAdd(&fr.stats[0], (unsigned long long)st_found);
-        if (st_fail) atomicAdd(&fr.stats[1], (unsigned long long)st_fail);
-        if (st_exp) atomicAdd(&fr.stats[2], (unsigned long long)st_exp);
-        if (st_disc) atomicAdd(&fr.stats[3], (unsigned long long)st_disc);
-        if (st_corr) atomicAdd(&fr.stats[4], (unsigned long long)st_corr);
-        if (st_rounds) atomicAdd(&fr.stats[5], (unsigned long long)st_rounds);
+        unsigned long long* const part = fr.stats_part + (blockIdx.x % kStatStripes) * 8;
+        if (st_found) atomicAdd(part + 0, (unsigned long long)st_found);
+        if (st_fail) atomicAdd(part + 1, (unsigned long long)st_fail);
+        if (st_exp) atomicAdd(part + 2, (unsigned long long)st_exp);
+        if (st_disc) atomicAdd(part + 3, (unsigned long long)st_disc);
+        if (st_corr) atomicAdd(part + 4, (unsigned long long)st_corr);
+        if (st_rounds) atomicAdd(part + 5, (unsigned long long)st_rounds);
     }
 }
 
