@@ -470,6 +470,9 @@ __global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
 // wait for all memory operations (the counter is in order), and the stores
 // would also hold up the end of the launch.
 // Thread i of a grid of at least max(N, 8) threads, in whole waves.
+// kFold: the frame ran a search or an admission, whose counters are in the
+// stripes (blocks of 256 threads; block 0 folds them).
+template <bool kFold>
 __device__ __forceinline__ void export_agent(const DevAgents& ag, const DevFrame& fr, int i) {
     const int s = i < ag.n ? ag.state[i] : 0;
     const bool changed = i < ag.n && s != fr.state_prev[i];
@@ -486,17 +489,33 @@ __device__ __forceinline__ void export_agent(const DevAgents& ag, const DevFrame
     const unsigned long long chg = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && i < ag.n)
         fr.h_group_out[i >> 6] = __popcll(srch) | (__popcll(mov) << 8) | (chg != 0ull ? 1 << 16 : 0);
-    if (i < 8) {
-        // the frame's additions, striped over kStatStripes sets of words,
-        // into the cumulative counters (and the stripes back to 0)
-        unsigned long long add = 0;
-#pragma unroll 16
-        for (int p = 0; p < kStatStripes; ++p) add += fr.stats_part[p * 8 + i];
-#pragma unroll 16
-        for (int p = 0; p < kStatStripes; ++p) fr.stats_part[p * 8 + i] = 0ull;
-        const unsigned long long t = fr.stats[i] + add;
-        fr.stats[i] = t;
-        fr.h_stats_out[i] = t;
+    if constexpr (kFold) {
+        static_assert(kStatStripes == 64, "the fold takes two stripes per thread of 256");
+        if (blockIdx.x == 0) {
+            // the frame's additions, striped over 64 sets of 8 words, into
+            // the cumulative counters (and the stripes back to 0): thread t
+            // takes counter t % 8 of stripes t / 8 and t / 8 + 32, then a
+            // tree over the threads of one counter
+            __shared__ unsigned long long red[256];
+            const int t = threadIdx.x;
+            const int a = t, b = t + 256;  // word p * 8 + c of stripes p and p + 32
+            const unsigned long long v = fr.stats_part[a] + fr.stats_part[b];
+            fr.stats_part[a] = 0ull;
+            fr.stats_part[b] = 0ull;
+            red[t] = v;
+            __syncthreads();
+            for (int s = 128; s >= 8; s >>= 1) {
+                if (t < s) red[t] += red[t + s];
+                __syncthreads();
+            }
+            if (t < 8) {
+                const unsigned long long tot = fr.stats[t] + red[t];
+                fr.stats[t] = tot;
+                fr.h_stats_out[t] = tot;
+            }
+        }
+    } else {
+        if (i < 8) fr.h_stats_out[i] = fr.stats[i];
     }
     if (i < 2) fr.h_slab_out[i] = ag.slab_top[i];  // [0] top, [1] deferred (adjacent words)
     // the movers of this frame are counted; the next frame counts from 0
@@ -506,7 +525,7 @@ __device__ __forceinline__ void export_agent(const DevAgents& ag, const DevFrame
 }
 
 __global__ void k_export(DevAgents ag, DevFrame fr) {
-    export_agent(ag, fr, blockIdx.x * blockDim.x + threadIdx.x);
+    export_agent<true>(ag, fr, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ------------------------------------------------------------- corridor slab
@@ -864,7 +883,7 @@ __global__ void k_pos_out(DevAgents ag, DevFrame fr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if constexpr (kExport) {
         if (i < ag.n) fr.h_pos_out[i] = ag.pos[i];
-        export_agent(ag, fr, i);
+        export_agent<false>(ag, fr, i);  // no search or admission ran
     } else {
         if (i >= ag.n || fr.counters[5] == 0) return;
         fr.h_pos_out[i] = ag.pos[i];
