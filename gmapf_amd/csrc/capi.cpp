@@ -80,8 +80,9 @@ struct gmapf_s {
     // most one per agent (a later SetAgentTargPos overwrites the entry in
     // place).  Between steps they are written straight into the next frame's
     // pinned input (h_upd, h_upd_xy): entry upd_pos[id] holds agent id's
-    // change when upd_epoch_of[id] == upd_epoch, and upd_n entries are in
-    // use; a finished step starts a new epoch, so clearing is O(1).  Changes
+    // change when upd_epoch_of[id] == upd_epoch, in the bucket of agent id's
+    // kUpdBucket agents; upd_n entries are in use, upd_buckets lists the
+    // buckets with a count to clear; a finished step starts a new epoch.  Changes
     // made while a step runs (its input is in use) queue in upd_ids / upd_targ
     // / upd_xy (upd_qpos[id]: index, or -1) and move to the pinned input when
     // the step ends.
@@ -89,6 +90,7 @@ struct gmapf_s {
     std::vector<unsigned> upd_epoch_of;
     unsigned upd_epoch = 1;
     int upd_n = 0;
+    std::vector<int> upd_buckets;
     std::vector<float> upd_xy;
 
     // device
@@ -129,9 +131,10 @@ struct gmapf_s {
     std::vector<int> host_dirty;
     std::vector<unsigned char> host_dirty_flag;
     long long crowd_frames = 0;  // frames that ran without admission and search
-    int* h_upd = nullptr;       // [0] count, then ids[N], targ[N]
+    int* h_upd = nullptr;       // bucket counts [ceil(N / kUpdBucket)], then ids[N], targ[N]
     float2* h_upd_xy = nullptr;
     int upd_cap = 0;
+    int upd_nb = 0;  // buckets of the update input
     // batched frame graphs, [compaction before the search][search timed]
     // [compaction before the search][search timed][crowd phase beside the search]
     hipGraph_t graph[2][2][2] = {};
@@ -516,9 +519,11 @@ struct gmapf_s {
         searching_after = -1;
         moving_after = -1;
         upd_cap = std::max(N, 1);
-        HIPCHK(hipHostMalloc((void**)&h_upd, (1 + 2 * (size_t)upd_cap) * sizeof(int)));
+        upd_nb = (upd_cap + kUpdBucket - 1) / kUpdBucket;
+        HIPCHK(hipHostMalloc((void**)&h_upd, ((size_t)upd_nb + 2 * (size_t)upd_cap) * sizeof(int)));
         HIPCHK(hipHostMalloc((void**)&h_upd_xy, (size_t)upd_cap * sizeof(float2)));
-        h_upd[0] = 0;
+        std::memset(h_upd, 0, (size_t)upd_nb * sizeof(int));
+        upd_buckets.clear();
         *h_step = 0.f;
         // the crowd's last kernel writes only the positions it sets, the
         // export pass only the states that changed
@@ -526,9 +531,9 @@ struct gmapf_s {
         if (N > 0) std::memcpy(h_state_stage, state.data(), (size_t)N * sizeof(int));
         movers = std::count(state.begin(), state.begin() + N, (int)GMAPF_STATE_MOVE);
         dfr.h_step = h_step;
-        dfr.h_upd_n = h_upd;
-        dfr.h_upd_ids = h_upd + 1;
-        dfr.h_upd_targ = h_upd + 1 + upd_cap;
+        dfr.h_upd_bcnt = h_upd;
+        dfr.h_upd_ids = h_upd + upd_nb;
+        dfr.h_upd_targ = h_upd + upd_nb + upd_cap;
         dfr.h_upd_xy = h_upd_xy;
         dfr.h_pos_out = reinterpret_cast<float2*>(h_pos_stage);
         dfr.h_state_out = h_state_stage;
@@ -655,7 +660,7 @@ struct gmapf_s {
         // frame inputs go to pinned host memory that the frame's first
         // kernels read directly (no copies; the captured graph stays fixed)
         *h_step = cfg.agent_speed * dt;  // AgentsMover.h:299
-        h_upd[0] = upd_n;  // the changes are already in place (stage_update)
+        // the changes are already in place (stage_update)
         // compact the corridor slab before this frame's search when the
         // last frame left it half full (the host reads the slab top that the
         // last frame exported; no device round trip)
@@ -732,6 +737,8 @@ struct gmapf_s {
         // changes made while the step ran stay pending for the next step
         upd_n = 0;
         upd_epoch++;
+        for (const int b : upd_buckets) h_upd[b] = 0;
+        upd_buckets.clear();
         movers = moving_after;
         for (size_t k = 0; k < upd_ids.size(); k++) {
             const int id = upd_ids[k];
@@ -849,12 +856,15 @@ struct gmapf_s {
         }
         int k = upd_pos[id];
         if (upd_epoch_of[id] != upd_epoch) {
-            k = upd_n++;
+            const int b = id / kUpdBucket;
+            if (h_upd[b] == 0) upd_buckets.push_back(b);
+            k = b * kUpdBucket + h_upd[b]++;
+            upd_n++;
             upd_pos[id] = k;
             upd_epoch_of[id] = upd_epoch;
-            h_upd[1 + k] = id;
+            h_upd[upd_nb + k] = id;
         }
-        h_upd[1 + upd_cap + k] = node;
+        h_upd[upd_nb + upd_cap + k] = node;
         h_upd_xy[k] = make_float2(x, y);
     }
 

@@ -116,6 +116,11 @@ struct DevSlots {
 // takes a few adds per launch.
 constexpr int kStatStripes = 64;
 
+// Agents per bucket of the target-update input (one admission workgroup's
+// agents, so one workgroup can apply its agents' changes and then admit
+// them, with no other workgroup involved).
+constexpr int kUpdBucket = 1024;
+
 struct DevFrame {
     int* blk;        // per-block request counts, then their exclusive prefixes
     int* mblk;       // per-block moving-agent counts
@@ -148,10 +153,13 @@ struct DevFrame {
     // pinned host memory read / written by the frame's kernels (no copies,
     // so a captured frame is one graph launch)
     const float* h_step;         // speed * dt of the frame
-    const int* h_upd_n;          // number of target updates
-    const int* h_upd_ids;        // [n] agent ids (at most one per agent)
-    const int* h_upd_targ;       // [n] target nodes
-    const float2* h_upd_xy;      // [n] target positions
+    // target updates, bucketed by kUpdBucket agents: bucket b's changes are
+    // entries b * kUpdBucket .. + h_upd_bcnt[b] - 1, each of an agent of b
+    // (at most one per agent)
+    const int* h_upd_bcnt;       // [ceil(N / kUpdBucket)] changes per bucket
+    const int* h_upd_ids;        // [N] agent ids
+    const int* h_upd_targ;       // [N] target nodes
+    const float2* h_upd_xy;      // [N] target positions
     float2* h_pos_out;           // [N] positions after the frame
     int* h_state_out;            // [N] states after the frame (written where they changed)
     int* state_prev;             // [N] device copy of h_state_out

@@ -370,11 +370,22 @@ __global__ __launch_bounds__(kAdmitThreads) void k_admit_scatter(DevAgents ag, D
 // Admission when every agent has a slot of its own (P >= N): every request
 // is admitted (PathFinder.h:85-116 with no deferral), into slot i for agent i
 // (PathFinder.h:104-115); the moving set is taken as in k_admit_count.  One
-// pass, no ranks; counters[5] (cleared by k_apply_updates) is set when some
-// agent moves, and the admissions are added to stats[6] through its stripes.
+// pass, no ranks, and the frame's first kernel: the workgroup first applies
+// the target changes of its own agents (their bucket of the input; a change
+// frees only the agent's own slot), then admits them.  counters[5] (0 since
+// the last export) is set when some agent moves, and the admissions are
+// added to stats[6] through its stripes.
+static_assert(kAdmitThreads == kUpdBucket, "a workgroup applies and admits one bucket's agents");
+__device__ __forceinline__ void apply_update(const DevAgents& ag, const DevSlots& sl, const DevFrame& fr, int k);
 __global__ __launch_bounds__(kAdmitThreads) void k_admit_identity(DevAgents ag, DevSlots sl, DevFrame fr) {
     __shared__ int wsum_m[kAdmitWaves], wsum_a[kAdmitWaves];
     const int i = blockIdx.x * kAdmitThreads + threadIdx.x;
+    if (i == 0) {
+        *fr.stepv = *fr.h_step;
+        fr.counters[2] = 0;  // search tickets
+    }
+    apply_update(ag, sl, fr, i);
+    __syncthreads();  // the workgroup's changes before its admission
     bool mv = false, adm = false;
     if (i < ag.n) {
         const int st = ag.state[i];
@@ -432,19 +443,12 @@ __global__ void k_admit_slots(DevAgents ag, DevSlots sl, DevFrame fr) {
 // host memory; the host passes at most one update per agent (the last one).
 // An agent whose search is in flight drops it: its slot is free for this
 // frame's admission, and the agent requests again with the new target.
-__global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) {
-        *fr.stepv = *fr.h_step;
-        fr.counters[2] = 0;  // search tickets (the ranked admission clears it too)
-        fr.counters[5] = 0;  // counted by the admission pass that follows
-        fr.counters[6] = 0;
-    }
-    // every lane reads the count (one request per wave), then the lanes
-    // below it read their entry: a frame with few updates reads little
-    // host memory
-    const int n = *fr.h_upd_n;
-    if (k >= n || k >= ag.n) return;
+// Entry k of the bucketed input (thread k): every lane reads its bucket's
+// count (one request per wave), then the lanes below it read their entry, so
+// a frame with few updates reads little host memory.
+__device__ __forceinline__ void apply_update(const DevAgents& ag, const DevSlots& sl, const DevFrame& fr, int k) {
+    const int n = fr.h_upd_bcnt[k / kUpdBucket];
+    if (k % kUpdBucket >= n || k >= ag.n) return;
     const int i = fr.h_upd_ids[k];
     const int tg = fr.h_upd_targ[k];
     const float2 xy = fr.h_upd_xy[k];
@@ -457,6 +461,17 @@ __global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
         sl.agent[s] = -1;
         ag.inflight[i] = -1;
     }
+}
+
+__global__ void k_apply_updates(DevAgents ag, DevSlots sl, DevFrame fr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
+        *fr.stepv = *fr.h_step;
+        fr.counters[2] = 0;  // search tickets (the ranked admission clears it too)
+        fr.counters[5] = 0;  // counted by the admission pass that follows
+        fr.counters[6] = 0;
+    }
+    apply_update(ag, sl, fr, k);
 }
 
 // States and counters of the finished frame into pinned host memory.  The
@@ -923,11 +938,11 @@ hipError_t launch_admit(const DevMap& map, const DevAgents& ag, const DevSlots& 
                         const DevFrame& fr, bool ident, hipStream_t st) {
     (void)map;
     const int nbA = nblk(ag.n, kAdmitThreads), nbS = nblk(sl.P, kAdmitThreads);
-    hipLaunchKernelGGL(k_apply_updates, dim3(nblk(ag.n > 0 ? ag.n : 1, 256)), dim3(256), 0, st, ag, sl, fr);
     if (ident) {
         hipLaunchKernelGGL(k_admit_identity, dim3(nbA > 0 ? nbA : 1), dim3(kAdmitThreads), 0, st, ag, sl, fr);
         return hipGetLastError();
     }
+    hipLaunchKernelGGL(k_apply_updates, dim3(nblk(ag.n > 0 ? ag.n : 1, 256)), dim3(256), 0, st, ag, sl, fr);
     hipLaunchKernelGGL(k_admit_count, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
     hipLaunchKernelGGL(k_admit_finish, dim3(1), dim3(kAdmitThreads), 0, st, sl, fr, nbA, nbS);
     hipLaunchKernelGGL(k_admit_scatter, dim3(nbA + nbS), dim3(kAdmitThreads), 0, st, ag, sl, fr, nbA);
